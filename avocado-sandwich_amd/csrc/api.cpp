@@ -1,0 +1,576 @@
+// api.cpp -- the C ABI of libavsvd.so (declared in include/avsvd.h).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "avsvd.h"
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "schedule.hpp"
+
+using namespace avs;
+
+namespace avs {
+
+thread_local std::string g_err;
+
+avs_status fail(avs_status code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+} // namespace avs
+
+#define HIPCHK(call)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) return fail(AVS_ERR_HIP, "%s failed: %s (%d)", #call, hipGetErrorString(e_), (int)e_); \
+    } while (0)
+
+namespace {
+
+bool supported_b(int b) { return b == 32 || b == 64 || b == 128; }
+size_t esize(int dtype) { return dtype == AVS_F64 ? 8 : 4; }
+
+template <typename F> avs_status dispatch_tb(int dtype, int b, F &&f)
+{
+    if (dtype == AVS_F64) {
+        if (b == 32) return f((double *)0, std::integral_constant<int, 32>());
+        if (b == 64) return f((double *)0, std::integral_constant<int, 64>());
+        if (b == 128) return f((double *)0, std::integral_constant<int, 128>());
+    } else if (dtype == AVS_F32) {
+        if (b == 32) return f((float *)0, std::integral_constant<int, 32>());
+        if (b == 64) return f((float *)0, std::integral_constant<int, 64>());
+        if (b == 128) return f((float *)0, std::integral_constant<int, 128>());
+    }
+    return fail(AVS_ERR_UNSUPPORTED, "dtype %d with tile size %d is not built", dtype, b);
+}
+
+struct DevProgram {
+    Program P;
+    Task *dTasks = nullptr;
+    int *dDepTile = nullptr, *dDepVal = nullptr, *dSig = nullptr;
+    void release()
+    {
+        hipFree(dTasks);
+        hipFree(dDepTile);
+        hipFree(dDepVal);
+        hipFree(dSig);
+        dTasks = nullptr;
+        dDepTile = dDepVal = dSig = nullptr;
+    }
+};
+
+} // namespace
+
+struct avs_handle {
+    int dtype = AVS_F64;
+    int64_t n = 0;
+    int b = 0, nt = 0;
+    avs_opts opts{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    void *dTiles = nullptr;
+    void *dStage = nullptr; // column-major staging (n*n elements)
+    double *dTau = nullptr, *dTjj = nullptr, *dBand = nullptr;
+    int *dCnt = nullptr, *dTicket = nullptr, *dStatus = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool have_matrix = false;
+    int last_alg = -1, last_kend = 0;
+    bool timed = false;
+    int workers = 0;
+    long long wallclock_khz = 100000;
+    DevProgram prog;
+    bool prog_valid = false;
+};
+
+namespace {
+
+int resolve_tree(const avs_handle *h) { return h->opts.tree; }
+int resolve_domain(const avs_handle *h)
+{
+    if (h->opts.domain > 0) return h->opts.domain;
+    // domain of ~16 tiles (tiles fewer than that -> flat)
+    return 16;
+}
+int upd_tiles_for(int b) { return b >= 128 ? 1 : (b == 64 ? 2 : 4); }
+
+avs_status ensure_program(avs_handle *h, int alg, int kend)
+{
+    const int tree = resolve_tree(h), dom = resolve_domain(h), ut = upd_tiles_for(h->b);
+    if (h->prog_valid && h->prog.P.alg == alg && h->prog.P.kend == kend && h->prog.P.tree == tree &&
+        h->prog.P.domain == dom)
+        return AVS_OK;
+    h->prog.release();
+    h->prog_valid = false;
+    build_program(h->prog.P, h->nt, h->b, alg, kend, tree, dom, ut, h->workers);
+    Program &P = h->prog.P;
+    if (h->opts.sched == AVS_SCHED_PERSISTENT) {
+        std::vector<Task> ordered(P.order.size());
+        for (size_t i = 0; i < P.order.size(); ++i) ordered[i] = P.prog[P.order[i]];
+        const size_t nd = std::max<size_t>(P.dep_tile.size(), 1), ns = std::max<size_t>(P.sig_tile.size(), 1);
+        HIPCHK(hipMalloc(&h->prog.dTasks, sizeof(Task) * std::max<size_t>(ordered.size(), 1)));
+        HIPCHK(hipMalloc(&h->prog.dDepTile, sizeof(int) * nd));
+        HIPCHK(hipMalloc(&h->prog.dDepVal, sizeof(int) * nd));
+        HIPCHK(hipMalloc(&h->prog.dSig, sizeof(int) * ns));
+        if (!ordered.empty())
+            HIPCHK(hipMemcpy(h->prog.dTasks, ordered.data(), sizeof(Task) * ordered.size(), hipMemcpyHostToDevice));
+        if (!P.dep_tile.empty()) {
+            HIPCHK(hipMemcpy(h->prog.dDepTile, P.dep_tile.data(), sizeof(int) * P.dep_tile.size(),
+                             hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->prog.dDepVal, P.dep_val.data(), sizeof(int) * P.dep_val.size(),
+                             hipMemcpyHostToDevice));
+        }
+        if (!P.sig_tile.empty())
+            HIPCHK(hipMemcpy(h->prog.dSig, P.sig_tile.data(), sizeof(int) * P.sig_tile.size(),
+                             hipMemcpyHostToDevice));
+    }
+    h->prog_valid = true;
+    return AVS_OK;
+}
+
+template <typename TS, int B> avs_status run_host_enqueued(avs_handle *h)
+{
+    const Program &P = h->prog.P;
+    const long tsz = (long)B * B;
+    TS *tiles = (TS *)h->dTiles;
+    size_t i = 0;
+    while (i < P.prog.size()) {
+        const Task &t = P.prog[i];
+        double *tau = h->dTau + (long)t.fac * B;
+        double *tjj = h->dTjj + (long)t.fac * 16 * B;
+        if (t.kind == TK_PANEL) {
+            launch_panel<TS, B>(t.mode, tiles + t.vtile * tsz, B, tiles + t.btile * tsz, B, t.trans, tau, tjj,
+                                h->stream);
+            ++i;
+            continue;
+        }
+        // group the consecutive update tasks of one panel op into one launch
+        size_t j = i;
+        int total = 0;
+        while (j < P.prog.size() && P.prog[j].kind == TK_UPDATE && P.prog[j].fac == t.fac) {
+            total += P.prog[j].ntiles;
+            ++j;
+        }
+        const long st = (long)t.cstep * tsz;
+        launch_update<TS, B>(t.mode, true, tiles + t.vtile * tsz, B, t.trans, tjj, tiles + t.c1tile * tsz, B, st,
+                             tiles + t.btile * tsz, B, st, t.trans, total, t.ntiles, h->stream);
+        i = j;
+    }
+    HIPCHK(hipGetLastError());
+    return AVS_OK;
+}
+
+template <typename TS, int B> avs_status run_persistent(avs_handle *h)
+{
+    const Program &P = h->prog.P;
+    HIPCHK(hipMemsetAsync(h->dCnt, 0, sizeof(int) * (size_t)h->nt * h->nt, h->stream));
+    HIPCHK(hipMemsetAsync(h->dTicket, 0, sizeof(int), h->stream));
+    HIPCHK(hipMemsetAsync(h->dStatus, 0, sizeof(int), h->stream));
+    // dependency wait bound: opts.reserved[0] milliseconds (default 30 s)
+    const long long ms = h->opts.reserved[0] > 0 ? h->opts.reserved[0] : 30000;
+    launch_persistent<TS, B>(h->workers, h->dTiles, h->dTau, h->dTjj, h->prog.dTasks, (int)P.order.size(),
+                             h->prog.dDepTile, h->prog.dDepVal, h->prog.dSig, h->dCnt, h->dTicket,
+                             ms * h->wallclock_khz, h->dStatus, h->stream);
+    HIPCHK(hipGetLastError());
+    return AVS_OK;
+}
+
+avs_status run_alg(avs_handle *h, int alg, int32_t kend)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    if (kend <= 0 || kend > h->nt) kend = h->nt;
+    avs_status st = ensure_program(h, alg, kend);
+    if (st) return st;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    st = dispatch_tb(h->dtype, h->b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int B = decltype(bc)::value;
+        if (h->opts.sched == AVS_SCHED_PERSISTENT) return run_persistent<TS, B>(h);
+        return run_host_enqueued<TS, B>(h);
+    });
+    if (st) return st;
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    h->timed = true;
+    h->last_alg = alg;
+    h->last_kend = kend;
+    return AVS_OK;
+}
+
+avs_status check_status(avs_handle *h)
+{
+    int s = 0;
+    HIPCHK(hipMemcpyAsync(&s, h->dStatus, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (s != 0)
+        return fail(AVS_ERR_STATE, "persistent executor: task %d exceeded its dependency wait bound", s - 1);
+    return AVS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *avs_last_error(void) { return g_err.c_str(); }
+
+void avs_default_opts(avs_opts *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->sched = AVS_SCHED_PERSISTENT;
+    o->tree = AVS_TREE_HIER;
+    o->domain = 0;
+    o->update_mode = AVS_UPD_NATIVE;
+}
+
+avs_status avs_create(avs_handle **hp, int32_t dtype, int64_t n, int32_t b, const avs_opts *opts)
+{
+    if (!hp) return fail(-1, "null handle pointer");
+    *hp = nullptr;
+    if (dtype != AVS_F64 && dtype != AVS_F32) return fail(-2, "dtype %d not supported", dtype);
+    if (n <= 0 || b <= 0 || n % b != 0) return fail(-3, "Not supported: n=%lld is not a positive multiple of b=%d",
+                                                    (long long)n, b);
+    if (!supported_b(b)) return fail(-4, "Not supported: tile size %d (built: 32, 64, 128)", b);
+    std::unique_ptr<avs_handle> h(new avs_handle());
+    h->dtype = dtype;
+    h->n = n;
+    h->b = b;
+    h->nt = (int)(n / b);
+    if (opts) h->opts = *opts;
+    else avs_default_opts(&h->opts);
+    if (h->opts.stream) h->stream = (hipStream_t)h->opts.stream;
+    else {
+        HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    const size_t nt2 = (size_t)h->nt * h->nt;
+    const size_t nslots = 4 * nt2;
+    HIPCHK(hipMalloc(&h->dTiles, esize(dtype) * (size_t)n * n));
+    HIPCHK(hipMalloc(&h->dTau, sizeof(double) * nslots * b));
+    HIPCHK(hipMalloc(&h->dTjj, sizeof(double) * nslots * 16 * b));
+    HIPCHK(hipMalloc(&h->dCnt, sizeof(int) * nt2));
+    HIPCHK(hipMalloc(&h->dTicket, sizeof(int)));
+    HIPCHK(hipMalloc(&h->dStatus, sizeof(int)));
+    HIPCHK(hipMemset(h->dStatus, 0, sizeof(int)));
+    HIPCHK(hipEventCreate(&h->ev0));
+    HIPCHK(hipEventCreate(&h->ev1));
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int wclk = 0;
+    if (hipDeviceGetAttribute(&wclk, hipDeviceAttributeWallClockRate, dev) == hipSuccess && wclk > 0)
+        h->wallclock_khz = wclk;
+    int occ = 0;
+    avs_status st = dispatch_tb(dtype, b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int B = decltype(bc)::value;
+        occ = persistent_occupancy<TS, B>();
+        return AVS_OK;
+    });
+    if (st) return st;
+    if (occ < 1) return fail(AVS_ERR_HIP, "persistent executor does not fit on a CU");
+    const int maxw = ncu * occ;
+    h->workers = h->opts.workers > 0 ? std::min(h->opts.workers, maxw) : maxw;
+    *hp = h.release();
+    return AVS_OK;
+}
+
+avs_status avs_destroy(avs_handle *h)
+{
+    if (!h) return AVS_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    h->prog.release();
+    hipFree(h->dTiles);
+    hipFree(h->dStage);
+    hipFree(h->dTau);
+    hipFree(h->dTjj);
+    hipFree(h->dBand);
+    hipFree(h->dCnt);
+    hipFree(h->dTicket);
+    hipFree(h->dStatus);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->own_stream) hipStreamDestroy(h->stream);
+    delete h;
+    return AVS_OK;
+}
+
+static avs_status ensure_stage(avs_handle *h)
+{
+    if (!h->dStage) HIPCHK(hipMalloc(&h->dStage, esize(h->dtype) * (size_t)h->n * h->n));
+    return AVS_OK;
+}
+
+avs_status avs_set_matrix_device(avs_handle *h, const void *dA, int64_t lda)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!dA) return fail(-2, "null matrix");
+    if (lda < h->n) return fail(-3, "lda < n");
+    if (h->dtype == AVS_F64) launch_pack<double>(dA, lda, h->dTiles, h->nt, h->b, h->stream);
+    else launch_pack<float>(dA, lda, h->dTiles, h->nt, h->b, h->stream);
+    HIPCHK(hipGetLastError());
+    h->have_matrix = true;
+    h->last_alg = -1;
+    return AVS_OK;
+}
+
+avs_status avs_set_matrix_host(avs_handle *h, const void *A, int64_t lda)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!A) return fail(-2, "null matrix");
+    if (lda < h->n) return fail(-3, "lda < n");
+    avs_status st = ensure_stage(h);
+    if (st) return st;
+    const size_t es = esize(h->dtype);
+    HIPCHK(hipMemcpy2DAsync(h->dStage, es * h->n, A, es * lda, es * h->n, h->n, hipMemcpyHostToDevice, h->stream));
+    return avs_set_matrix_device(h, h->dStage, h->n);
+}
+
+avs_status avs_get_matrix_device(avs_handle *h, void *dA, int64_t lda)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!dA) return fail(-2, "null matrix");
+    if (lda < h->n) return fail(-3, "lda < n");
+    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    const int mode = h->last_alg == ALG_BLOCKBIDIAG ? 1 : 0;
+    const int kend = h->last_alg == ALG_BLOCKBIDIAG ? h->last_kend : 0;
+    if (h->dtype == AVS_F64) launch_unpack<double>(h->dTiles, dA, lda, h->nt, h->b, mode, kend, h->stream);
+    else launch_unpack<float>(h->dTiles, dA, lda, h->nt, h->b, mode, kend, h->stream);
+    HIPCHK(hipGetLastError());
+    return AVS_OK;
+}
+
+avs_status avs_get_matrix_host(avs_handle *h, void *A, int64_t lda)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!A) return fail(-2, "null matrix");
+    if (lda < h->n) return fail(-3, "lda < n");
+    avs_status st = ensure_stage(h);
+    if (st) return st;
+    st = avs_get_matrix_device(h, h->dStage, h->n);
+    if (st) return st;
+    const size_t es = esize(h->dtype);
+    HIPCHK(hipMemcpy2DAsync(A, es * lda, h->dStage, es * h->n, es * h->n, h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return check_status(h);
+}
+
+avs_status avs_blockbidiag(avs_handle *h, int32_t kend) { return run_alg(h, ALG_BLOCKBIDIAG, kend); }
+avs_status avs_tiled_qr(avs_handle *h, int32_t kend) { return run_alg(h, ALG_TILED_QR, kend); }
+
+avs_status avs_synchronize(avs_handle *h)
+{
+    if (!h) return fail(-1, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return check_status(h);
+}
+
+avs_status avs_last_time_ms(avs_handle *h, double *ms)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!ms) return fail(-2, "null output");
+    if (!h->timed) return fail(AVS_ERR_STATE, "nothing timed yet");
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, h->ev0, h->ev1));
+    *ms = f;
+    return check_status(h);
+}
+
+avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!band) return fail(-2, "null band");
+    if (ldband < h->b + 1) return fail(-3, "ldband < b+1");
+    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    if (!h->dBand) HIPCHK(hipMalloc(&h->dBand, sizeof(double) * (size_t)(h->b + 1) * h->n));
+    if (h->dtype == AVS_F64) launch_band<double>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
+    else launch_band<float>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy2DAsync(band, sizeof(double) * ldband, h->dBand, sizeof(double) * (h->b + 1),
+                            sizeof(double) * (h->b + 1), h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return check_status(h);
+}
+
+avs_status avs_svdvals(avs_handle *h, double *sigma)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!sigma) return fail(-2, "null sigma");
+    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    avs_status st;
+    if (h->last_alg != ALG_BLOCKBIDIAG || h->last_kend != h->nt) {
+        st = avs_blockbidiag(h, 0);
+        if (st) return st;
+    }
+    const int64_t n = h->n;
+    const int b = h->b;
+    std::vector<double> band((size_t)(b + 1) * n), d(n), e(n > 1 ? n - 1 : 1);
+    st = avs_get_band(h, band.data(), b + 1);
+    if (st) return st;
+    st = avs_bidiagonalize_band(n, b, band.data(), b + 1, d.data(), e.data());
+    if (st) return st;
+    return avs_bdsdc_values(n, d.data(), e.data(), sigma);
+}
+
+// ------------------------------------------------------------ tile operators
+avs_status avs_geqrt(int32_t dtype, int32_t b, void *dA, int64_t lda, double *dtau, int32_t trans, void *stream)
+{
+    if (!dA) return fail(-3, "null tile");
+    if (lda < b) return fail(-4, "lda < b");
+    if (!dtau) return fail(-5, "null tau");
+    return dispatch_tb(dtype, b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int B = decltype(bc)::value;
+        launch_panel<TS, B>(MODE_GE, dA, lda, dA, lda, trans, dtau, nullptr, (hipStream_t)stream);
+        HIPCHK(hipGetLastError());
+        return AVS_OK;
+    });
+}
+
+avs_status avs_tsqrt(int32_t dtype, int32_t b, void *dA1, int64_t lda1, void *dA2, int64_t lda2, double *dtau,
+                     int32_t trans, void *stream)
+{
+    if (!dA1) return fail(-3, "null tile");
+    if (lda1 < b) return fail(-4, "lda1 < b");
+    if (!dA2) return fail(-5, "null tile");
+    if (lda2 < b) return fail(-6, "lda2 < b");
+    if (!dtau) return fail(-7, "null tau");
+    return dispatch_tb(dtype, b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int B = decltype(bc)::value;
+        launch_panel<TS, B>(MODE_TS, dA1, lda1, dA2, lda2, trans, dtau, nullptr, (hipStream_t)stream);
+        HIPCHK(hipGetLastError());
+        return AVS_OK;
+    });
+}
+
+static avs_status apply_op(int mode, int32_t dtype, int32_t b, const void *dV, int64_t ldv, const double *dtau,
+                           void *dC1, int64_t ldc1, void *dC2, int64_t ldc2, int64_t ncols, int32_t applyt,
+                           int32_t vtrans, int32_t ctrans, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const int ntl = (int)(ncols / b);
+    return dispatch_tb(dtype, b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int B = decltype(bc)::value;
+        double *dT = nullptr;
+        HIPCHK(hipMalloc(&dT, sizeof(double) * 16 * B));
+        launch_tjj<TS, B>(mode, dV, ldv, vtrans, dtau, dT, s);
+        const long s1 = ctrans ? (long)B : (long)B * ldc1;
+        const long s2 = ctrans ? (long)B : (long)B * ldc2;
+        launch_update<TS, B>(mode, applyt != 0, dV, ldv, vtrans, dT, dC1, ldc1, s1, dC2, ldc2, s2, ctrans, ntl,
+                             upd_tiles_for(B), s);
+        hipError_t e = hipGetLastError();
+        hipStreamSynchronize(s);
+        hipFree(dT);
+        if (e != hipSuccess) return fail(AVS_ERR_HIP, "update launch failed: %s", hipGetErrorString(e));
+        return AVS_OK;
+    });
+}
+
+avs_status avs_unmqr(int32_t dtype, int32_t b, const void *dV, int64_t ldv, const double *dtau, void *dC,
+                     int64_t ldc, int64_t ncols, int32_t applyt, int32_t vtrans, int32_t ctrans, void *stream)
+{
+    if (!dV) return fail(-3, "null V");
+    if (ldv < b) return fail(-4, "ldv < b");
+    if (!dtau) return fail(-5, "null tau");
+    if (!dC) return fail(-6, "null C");
+    if (!ctrans && ldc < b) return fail(-7, "ldc < b");
+    if (ncols < 0 || ncols % b != 0) return fail(-8, "ncols must be a multiple of b");
+    return apply_op(MODE_GE, dtype, b, dV, ldv, dtau, dC, ldc, dC, ldc, ncols, applyt, vtrans, ctrans, stream);
+}
+
+avs_status avs_tsmqr(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC2, int64_t ldc2, const void *dV,
+                     int64_t ldv, const double *dtau, int64_t ncols, int32_t applyt, int32_t vtrans, int32_t ctrans,
+                     void *stream)
+{
+    if (!dC1) return fail(-3, "null C1");
+    if (!dC2) return fail(-5, "null C2");
+    if (!dV) return fail(-7, "null V");
+    if (ldv < b) return fail(-8, "ldv < b");
+    if (!dtau) return fail(-9, "null tau");
+    if (ncols < 0 || ncols % b != 0) return fail(-10, "ncols must be a multiple of b");
+    return apply_op(MODE_TS, dtype, b, dV, ldv, dtau, dC1, ldc1, dC2, ldc2, ncols, applyt, vtrans, ctrans, stream);
+}
+
+avs_status avs_debug_mfma_layout(double *out64, float *out32)
+{
+    if (!out64 || !out32) return fail(-1, "null output");
+    double ha[64], hb[64];
+    float fa[64], fb[64];
+    for (int i = 0; i < 64; ++i) {
+        ha[i] = (double)(i + 1);
+        hb[i] = (double)((i * 7) % 13 - 6);
+        fa[i] = (float)ha[i];
+        fb[i] = (float)hb[i];
+    }
+    double *d = nullptr;
+    float *f = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * (128 + 256)));
+    HIPCHK(hipMalloc(&f, sizeof(float) * (128 + 256)));
+    HIPCHK(hipMemcpy(d, ha, sizeof ha, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + 64, hb, sizeof hb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f, fa, sizeof fa, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(f + 64, fb, sizeof fb, hipMemcpyHostToDevice));
+    launch_mfma_probe(d, d + 64, d + 128, f, f + 64, f + 128, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out64, d + 128, sizeof(double) * 256, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out32, f + 128, sizeof(float) * 256, hipMemcpyDeviceToHost));
+    hipFree(d);
+    hipFree(f);
+    return AVS_OK;
+}
+
+avs_status avs_schedule_stats(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
+                              int32_t workers, int64_t *ntasks, double *est_us)
+{
+    if (nt <= 0) return fail(-1, "nt must be positive");
+    if (b <= 0) return fail(-2, "b must be positive");
+    Program P;
+    build_program(P, nt, b, alg, kend <= 0 ? nt : kend, tree, domain, upd_tiles_for(b), workers);
+    if (ntasks) *ntasks = (int64_t)P.prog.size();
+    if (est_us) *est_us = P.est_makespan_us;
+    return AVS_OK;
+}
+
+avs_status avs_schedule_dump(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
+                             int32_t workers, int32_t *out, int64_t cap, int64_t *needed)
+{
+    if (nt <= 0) return fail(-1, "nt must be positive");
+    if (b <= 0) return fail(-2, "b must be positive");
+    if (!needed) return fail(-10, "null needed");
+    Program P;
+    build_program(P, nt, b, alg, kend <= 0 ? nt : kend, tree, domain, upd_tiles_for(b), workers);
+    std::vector<int32_t> v;
+    for (int idx : P.order) {
+        const Task &t = P.prog[idx];
+        const int32_t head[] = {t.kind, t.mode, t.trans, t.ntiles, t.vtile, t.btile, t.c1tile, t.cstep, t.fac, t.ndep};
+        v.insert(v.end(), head, head + 10);
+        for (int d = 0; d < t.ndep; ++d) {
+            v.push_back(P.dep_tile[t.dep0 + d]);
+            v.push_back(P.dep_val[t.dep0 + d]);
+        }
+        v.push_back(t.nsig);
+        for (int s2 = 0; s2 < t.nsig; ++s2) v.push_back(P.sig_tile[t.sig0 + s2]);
+    }
+    *needed = (int64_t)v.size();
+    if (out && cap >= (int64_t)v.size()) memcpy(out, v.data(), sizeof(int32_t) * v.size());
+    return AVS_OK;
+}
+
+} // extern "C"

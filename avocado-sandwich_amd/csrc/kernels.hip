@@ -1,0 +1,311 @@
+// kernels.hip -- launchable kernels around the tile-operator bodies of
+// device.hpp: standalone operator kernels (the host-enqueued plain path and the
+// tile-operator ABI), layout conversion, band extraction and the persistent
+// dependency-counting executor of the tile DAG.
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace avs {
+
+// ----------------------------------------------------------------- layout
+// column-major (lda) <-> tile-major: tile (i,j) at (j*nt+i)*b*b, column-major inside
+template <typename TS>
+__global__ void __launch_bounds__(256) pack_kernel(const TS *__restrict__ A, long lda, TS *__restrict__ tiles,
+                                                   int nt, int b)
+{
+    const int i = blockIdx.x, j = blockIdx.y;
+    TS *dst = tiles + ((long)j * nt + i) * b * b;
+    const TS *src = A + (long)i * b + (long)j * b * lda;
+    for (int idx = threadIdx.x; idx < b * b; idx += blockDim.x) {
+        const int r = idx % b, c = idx / b;
+        dst[idx] = src[r + (long)c * lda];
+    }
+}
+
+// mode 0: raw tiles; mode 1: the reference's Blockbidiag! output (band of the
+// first kend steps, exact zeros outside it, raw trailing block)
+template <typename TS>
+__global__ void __launch_bounds__(256) unpack_kernel(const TS *__restrict__ tiles, TS *__restrict__ A, long lda,
+                                                     int nt, int b, int mode, int kend)
+{
+    const int i = blockIdx.x, j = blockIdx.y;
+    const TS *src = tiles + ((long)j * nt + i) * b * b;
+    TS *dst = A + (long)i * b + (long)j * b * lda;
+    const bool banded = mode == 1 && (i < kend || j < kend);
+    for (int idx = threadIdx.x; idx < b * b; idx += blockDim.x) {
+        const int r = idx % b, c = idx / b;
+        TS v = src[idx];
+        if (banded) {
+            if (j == i) v = (r <= c) ? v : (TS)0;
+            else if (j == i + 1) v = (r >= c) ? v : (TS)0;
+            else v = (TS)0;
+        }
+        dst[r + (long)c * lda] = v;
+    }
+}
+
+// band[(b + i - j) + j*ldband] = A(i,j), max(0,j-b) <= i <= j, as FP64
+template <typename TS>
+__global__ void __launch_bounds__(256) band_kernel(const TS *__restrict__ tiles, double *__restrict__ band,
+                                                   long ldband, int nt, int b, long n)
+{
+    const long j = blockIdx.x;
+    for (int d = threadIdx.x; d <= b; d += blockDim.x) {
+        const long i = j - b + d;
+        double v = 0.0;
+        if (i >= 0) {
+            const long I = i / b, J = j / b;
+            const int r = (int)(i % b), c = (int)(j % b);
+            v = (double)tiles[(J * nt + I) * b * b + r + (long)c * b];
+        }
+        band[d + j * ldband] = v;
+    }
+}
+
+// ----------------------------------------------------- standalone operators
+template <typename TS, int B, int MODE>
+__global__ void __launch_bounds__(NTHREADS) panel_kernel(TS *top, long ldt, TS *bot, long ldb, int tr,
+                                                         double *tau, double *tjj)
+{
+    __shared__ double smem[Lds<TS, B>::panel];
+    panel_body<TS, B, MODE>(top, ldt, bot, ldb, tr != 0, tau, tjj, smem);
+}
+
+template <typename TS, int B, int MODE>
+__global__ void __launch_bounds__(NTHREADS) tjj_kernel(const TS *V, long ldv, int vtr, const double *tau,
+                                                       double *tjj)
+{
+    __shared__ double smem[Lds<TS, B>::tjj];
+    tjj_body<TS, B, MODE>(V, ldv, vtr != 0, tau, tjj, smem);
+}
+
+// one block per group of `ntiles` column tiles; group g starts at tile g*ntiles
+template <typename TS, int B, int MODE, bool APPLYT>
+__global__ void __launch_bounds__(NTHREADS) update_kernel(const TS *V, long ldv, int vtr, const double *tjj,
+                                                          TS *C1, long ld1, long s1, TS *C2, long ld2, long s2,
+                                                          int ctr, int ntiles_total, int per_block)
+{
+    __shared__ double smem[Lds<TS, B>::update];
+    const int first = blockIdx.x * per_block;
+    int cnt = ntiles_total - first;
+    if (cnt > per_block) cnt = per_block;
+    if (cnt <= 0) return;
+    update_body<TS, B, MODE, APPLYT>(V, ldv, vtr != 0, tjj, C1 + (long)first * s1, ld1, s1, C2 + (long)first * s2,
+                                     ld2, s2, ctr != 0, cnt, smem);
+}
+
+// ------------------------------------------------------ persistent executor
+__device__ __forceinline__ int ld_acquire(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename TS, int B>
+__global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, double *taus, double *tjjs,
+                                                                  const Task *__restrict__ tasks, int ntasks,
+                                                                  const int *__restrict__ dep_tile,
+                                                                  const int *__restrict__ dep_val,
+                                                                  const int *__restrict__ sig_tile, int *cnt,
+                                                                  int *ticket, long long wait_ticks, int *status)
+{
+    __shared__ double smem[Lds<TS, B>::maxv];
+    __shared__ int sh_task;
+    __shared__ int sh_fail;
+    const int t = threadIdx.x;
+    const long tsz = (long)B * B;
+    for (;;) {
+        if (t == 0) {
+            sh_task = atomicAdd(ticket, 1);
+            sh_fail = 0;
+        }
+        __syncthreads();
+        const int tk = sh_task;
+        if (tk >= ntasks) break;
+        const Task T = tasks[tk];
+        // wait for dependencies, bounded in wall-clock time so that a schedule
+        // error ends the launch with a status instead of hanging it
+        if (t == 0) {
+            const long long t0 = wall_clock64();
+            for (int d = 0; d < T.ndep; ++d) {
+                const int *p = cnt + dep_tile[T.dep0 + d];
+                const int need = dep_val[T.dep0 + d];
+                while (ld_acquire(p) < need) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > wait_ticks) {
+                        sh_fail = 1;
+                        break;
+                    }
+                }
+                if (sh_fail) break;
+            }
+        }
+        __syncthreads();
+        if (sh_fail) {
+            if (t == 0) atomicExch(status, 1 + tk);
+            break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        double *tau = taus + (long)T.fac * B;
+        double *tjj = tjjs + (long)T.fac * 16 * B;
+        const bool tr = T.trans != 0;
+        if (T.kind == TK_PANEL) {
+            TS *top = tiles + (long)T.vtile * tsz;
+            TS *bot = tiles + (long)T.btile * tsz;
+            if (T.mode == MODE_GE) panel_body<TS, B, MODE_GE>(top, B, top, B, tr, tau, tjj, smem);
+            else if (T.mode == MODE_TS) panel_body<TS, B, MODE_TS>(top, B, bot, B, tr, tau, tjj, smem);
+            else panel_body<TS, B, MODE_TT>(top, B, bot, B, tr, tau, tjj, smem);
+        } else {
+            const TS *V = tiles + (long)T.vtile * tsz;
+            TS *C2 = tiles + (long)T.btile * tsz;
+            TS *C1 = tiles + (long)T.c1tile * tsz;
+            const long st = (long)T.cstep * tsz;
+            if (T.mode == MODE_GE)
+                update_body<TS, B, MODE_GE, true>(V, B, tr, tjj, C2, B, st, C2, B, st, tr, T.ntiles, smem);
+            else if (T.mode == MODE_TS)
+                update_body<TS, B, MODE_TS, true>(V, B, tr, tjj, C1, B, st, C2, B, st, tr, T.ntiles, smem);
+            else
+                update_body<TS, B, MODE_TT, true>(V, B, tr, tjj, C1, B, st, C2, B, st, tr, T.ntiles, smem);
+        }
+        __syncthreads();
+        if (t == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            for (int s = 0; s < T.nsig; ++s)
+                __hip_atomic_fetch_add(cnt + sig_tile[T.sig0 + s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// ------------------------------------------------------------ debug probes
+// One 16x16x4 MFMA per element type with A = a[i + 16k], B = b[k + 4j]; writes
+// the accumulator of every lane so the host can check the lane layout.
+__global__ void mfma_probe_kernel(const double *a, const double *b, double *out64, const float *af,
+                                  const float *bf, float *out32)
+{
+    const int l = threadIdx.x;
+    const int i = l & 15, k = l >> 4;
+    d4_t acc = vzero4<d4_t>();
+    acc = mfma16(a[i + 16 * k], b[k + 4 * i], acc);
+    for (int m = 0; m < 4; ++m) out64[l * 4 + m] = acc[m];
+    f4_t accf = vzero4<f4_t>();
+    accf = mfma16(af[i + 16 * k], bf[k + 4 * i], accf);
+    for (int m = 0; m < 4; ++m) out32[l * 4 + m] = accf[m];
+}
+
+// ------------------------------------------------------------ host launchers
+template <typename TS> void launch_pack(const void *A, long lda, void *tiles, int nt, int b, hipStream_t s)
+{
+    hipLaunchKernelGGL(pack_kernel<TS>, dim3(nt, nt), dim3(256), 0, s, (const TS *)A, lda, (TS *)tiles, nt, b);
+}
+template <typename TS>
+void launch_unpack(const void *tiles, void *A, long lda, int nt, int b, int mode, int kend, hipStream_t s)
+{
+    hipLaunchKernelGGL(unpack_kernel<TS>, dim3(nt, nt), dim3(256), 0, s, (const TS *)tiles, (TS *)A, lda, nt, b,
+                       mode, kend);
+}
+template <typename TS> void launch_band(const void *tiles, double *band, long ldband, int nt, int b, hipStream_t s)
+{
+    hipLaunchKernelGGL(band_kernel<TS>, dim3((unsigned)((long)nt * b)), dim3(256), 0, s, (const TS *)tiles, band,
+                       ldband, nt, b, (long)nt * b);
+}
+
+template <typename TS, int B>
+void launch_panel(int mode, void *top, long ldt, void *bot, long ldb, int tr, double *tau, double *tjj, hipStream_t s)
+{
+    if (mode == MODE_GE)
+        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_GE>), dim3(1), dim3(NTHREADS), 0, s, (TS *)top, ldt, (TS *)top,
+                           ldt, tr, tau, tjj);
+    else if (mode == MODE_TS)
+        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_TS>), dim3(1), dim3(NTHREADS), 0, s, (TS *)top, ldt, (TS *)bot,
+                           ldb, tr, tau, tjj);
+    else
+        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_TT>), dim3(1), dim3(NTHREADS), 0, s, (TS *)top, ldt, (TS *)bot,
+                           ldb, tr, tau, tjj);
+}
+
+template <typename TS, int B>
+void launch_tjj(int mode, const void *V, long ldv, int vtr, const double *tau, double *tjj, hipStream_t s)
+{
+    if (mode == MODE_GE)
+        hipLaunchKernelGGL((tjj_kernel<TS, B, MODE_GE>), dim3(1), dim3(NTHREADS), 0, s, (const TS *)V, ldv, vtr, tau,
+                           tjj);
+    else if (mode == MODE_TS)
+        hipLaunchKernelGGL((tjj_kernel<TS, B, MODE_TS>), dim3(1), dim3(NTHREADS), 0, s, (const TS *)V, ldv, vtr, tau,
+                           tjj);
+    else
+        hipLaunchKernelGGL((tjj_kernel<TS, B, MODE_TT>), dim3(1), dim3(NTHREADS), 0, s, (const TS *)V, ldv, vtr, tau,
+                           tjj);
+}
+
+template <typename TS, int B, int MODE>
+static void launch_update_m(bool applyt, const void *V, long ldv, int vtr, const double *tjj, void *C1, long ld1,
+                            long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, hipStream_t s)
+{
+    const int nblk = (ntiles + per_block - 1) / per_block;
+    if (nblk <= 0) return;
+    if (applyt)
+        hipLaunchKernelGGL((update_kernel<TS, B, MODE, true>), dim3(nblk), dim3(NTHREADS), 0, s, (const TS *)V, ldv,
+                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block);
+    else
+        hipLaunchKernelGGL((update_kernel<TS, B, MODE, false>), dim3(nblk), dim3(NTHREADS), 0, s, (const TS *)V, ldv,
+                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block);
+}
+
+template <typename TS, int B>
+void launch_update(int mode, bool applyt, const void *V, long ldv, int vtr, const double *tjj, void *C1, long ld1,
+                   long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, hipStream_t s)
+{
+    if (mode == MODE_GE)
+        launch_update_m<TS, B, MODE_GE>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block, s);
+    else if (mode == MODE_TS)
+        launch_update_m<TS, B, MODE_TS>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block, s);
+    else
+        launch_update_m<TS, B, MODE_TT>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block, s);
+}
+
+template <typename TS, int B>
+void launch_persistent(int workers, void *tiles, double *taus, double *tjjs, const Task *tasks, int ntasks,
+                       const int *dep_tile, const int *dep_val, const int *sig_tile, int *cnt, int *ticket,
+                       long long wait_ticks, int *status, hipStream_t s)
+{
+    hipLaunchKernelGGL((persistent_kernel<TS, B>), dim3(workers), dim3(NTHREADS), 0, s, (TS *)tiles, taus, tjjs, tasks,
+                       ntasks, dep_tile, dep_val, sig_tile, cnt, ticket, wait_ticks, status);
+}
+
+template <typename TS, int B> int persistent_occupancy()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, persistent_kernel<TS, B>, NTHREADS, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+void launch_mfma_probe(const double *a, const double *b, double *o64, const float *af, const float *bf, float *o32,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, a, b, o64, af, bf, o32);
+}
+
+#define AVS_INST_B(TS, B)                                                                                             \
+    template void launch_panel<TS, B>(int, void *, long, void *, long, int, double *, double *, hipStream_t);         \
+    template void launch_tjj<TS, B>(int, const void *, long, int, const double *, double *, hipStream_t);             \
+    template void launch_update<TS, B>(int, bool, const void *, long, int, const double *, void *, long, long,        \
+                                       void *, long, long, int, int, int, hipStream_t);                                \
+    template void launch_persistent<TS, B>(int, void *, double *, double *, const Task *, int, const int *,           \
+                                           const int *, const int *, int *, int *, long long, int *, hipStream_t);    \
+    template int persistent_occupancy<TS, B>();
+
+AVS_INST_B(double, 32)
+AVS_INST_B(double, 64)
+AVS_INST_B(double, 128)
+AVS_INST_B(float, 32)
+AVS_INST_B(float, 64)
+AVS_INST_B(float, 128)
+
+template void launch_pack<double>(const void *, long, void *, int, int, hipStream_t);
+template void launch_pack<float>(const void *, long, void *, int, int, hipStream_t);
+template void launch_unpack<double>(const void *, void *, long, int, int, int, int, hipStream_t);
+template void launch_unpack<float>(const void *, void *, long, int, int, int, int, hipStream_t);
+template void launch_band<double>(const void *, double *, long, int, int, hipStream_t);
+template void launch_band<float>(const void *, double *, long, int, int, hipStream_t);
+
+} // namespace avs

@@ -1,0 +1,32 @@
+// kernels.hpp -- host-side declarations of the kernel launchers (kernels.hip).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+namespace avs {
+
+struct Task;
+
+template <typename TS> void launch_pack(const void *A, long lda, void *tiles, int nt, int b, hipStream_t s);
+template <typename TS>
+void launch_unpack(const void *tiles, void *A, long lda, int nt, int b, int mode, int kend, hipStream_t s);
+template <typename TS> void launch_band(const void *tiles, double *band, long ldband, int nt, int b, hipStream_t s);
+
+template <typename TS, int B>
+void launch_panel(int mode, void *top, long ldt, void *bot, long ldb, int tr, double *tau, double *tjj,
+                  hipStream_t s);
+template <typename TS, int B>
+void launch_tjj(int mode, const void *V, long ldv, int vtr, const double *tau, double *tjj, hipStream_t s);
+template <typename TS, int B>
+void launch_update(int mode, bool applyt, const void *V, long ldv, int vtr, const double *tjj, void *C1, long ld1,
+                   long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, hipStream_t s);
+template <typename TS, int B>
+void launch_persistent(int workers, void *tiles, double *taus, double *tjjs, const Task *tasks, int ntasks,
+                       const int *dep_tile, const int *dep_val, const int *sig_tile, int *cnt, int *ticket,
+                       long long wait_ticks, int *status, hipStream_t s);
+template <typename TS, int B> int persistent_occupancy();
+
+void launch_mfma_probe(const double *a, const double *b, double *o64, const float *af, const float *bf, float *o32,
+                       hipStream_t s);
+
+} // namespace avs

@@ -1,0 +1,28 @@
+// schedule.hpp -- tile-DAG construction for tiled_QR! / Blockbidiag!.
+#pragma once
+#include <vector>
+
+#include "task.hpp"
+
+namespace avs {
+
+enum { ALG_BLOCKBIDIAG = 0, ALG_TILED_QR = 1 };
+
+struct Program {
+    int nt = 0, B = 0, alg = 0, kend = 0, tree = 0, domain = 0, upd_tiles = 1;
+    std::vector<Task> prog;  // program order (a valid sequential order)
+    std::vector<int> order;  // dispatch order for the persistent executor (indices into prog)
+    std::vector<int> dep_tile, dep_val, sig_tile;
+    int nfactor_slots = 0;
+    double est_makespan_us = 0.0; // list-scheduling estimate
+    long n_panel = 0, n_update = 0;
+};
+
+// Builds the task DAG of Blockbidiag! (tiledalgos.jl:17-34) or tiled_QR!
+// (tiledalgos.jl:6-15) over an nt x nt tile grid, for kend steps.
+// tree: 0 = flat (reference), 1 = hierarchical with `domain` tiles per domain.
+// upd_tiles: column tiles per update task.  workers: executor width used by
+// the list-scheduling simulation that fixes the dispatch order.
+void build_program(Program &P, int nt, int B, int alg, int kend, int tree, int domain, int upd_tiles, int workers);
+
+} // namespace avs

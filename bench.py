@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""bench.py -- n=16384 FP64 band reduction (Blockbidiag!) on one MI355X.
+
+One step = one two-sided tiled QR/LQ reduction of a dense n x n matrix to
+block-bidiagonal form (the reference's Blockbidiag!, KAbasedSVD/src/
+tiledalgos.jl:17-34): repack the pristine column-major input (resident in HBM)
+into the tile-major working copy, then run the tile DAG.  Algorithmic work per
+step = (8/3) n^3 flops (the gebrd count; T factors, tree merges and padding
+are not counted).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384] [--b 128]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "avocado-sandwich_amd"))
+
+METRIC = "n=16384 band-reduction TFLOP/s (% peak); mixed-size svdvals/s, p99 latency"
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
+SEED = 20261203
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, b):
+    """The oracle's C restatement of the reference's Blockbidiag! (one core),
+    timed on a bounded sample: one full reduction at size n."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    a = oracle.gaussian_matrix(n, SEED)
+    t0 = time.perf_counter()
+    oracle.blockbidiag(a, b)
+    dt = time.perf_counter() - t0
+    return {"value": (8.0 / 3.0) * n ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+            "sample": "oracle Blockbidiag! (reference loop order, FP64, 1 thread) on one n=%d b=%d Gaussian "
+                      "matrix: %.2f s" % (n, b, dt), "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--b", type=int, default=128)
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--sched", choices=["persistent", "host"], default="persistent")
+    ap.add_argument("--tree", choices=["hier", "flat"], default="hier")
+    ap.add_argument("--domain", type=int, default=0)
+    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check-n", type=int, default=4096,
+                    help="size of the sigma check against the cached FP64 dgesdd vector (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only (this engine runs one matrix per GPU; see the sharding design "
+              "in DESIGN.md); got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import avsvd
+    from avsvd._lib import Opts, check, dptr, lib
+
+    L = lib()
+    n, b = args.n, args.b
+    dt = np.float64 if args.dtype == "f64" else np.float32
+    code = avsvd.AVS_F64 if dt == np.float64 else avsvd.AVS_F32
+    dev = torch.device("cuda:0")
+
+    log("generating n=%d %s Gaussian (seed %d)" % (n, args.dtype, SEED))
+    A = avsvd.gaussian_matrix(n, SEED, dtype=dt)
+    fro = float(np.linalg.norm(A.astype(np.float64)))
+    dA = torch.from_numpy(A.T).to(dev)  # row-major (n,n) of A^T == column-major A
+    torch.cuda.synchronize()
+
+    o = Opts()
+    L.avs_default_opts(ctypes.byref(o))
+    o.sched = avsvd.SCHED_PERSISTENT if args.sched == "persistent" else avsvd.SCHED_HOST
+    o.tree = avsvd.TREE_HIER if args.tree == "hier" else avsvd.TREE_FLAT
+    o.domain = args.domain
+    o.reserved[0] = 120000
+    h = ctypes.c_void_p()
+    check(L.avs_create(ctypes.byref(h), code, n, b, ctypes.byref(o)))
+    pdA = ctypes.c_void_p(dA.data_ptr())
+
+    def step():
+        check(L.avs_set_matrix_device(h, pdA, n))
+        check(L.avs_blockbidiag(h, 0))
+
+    ms = ctypes.c_double()
+    t_build = time.perf_counter()
+    for w in range(args.warmup):
+        step()
+        check(L.avs_synchronize(h))
+        check(L.avs_last_time_ms(h, ctypes.byref(ms)))
+        log("warmup %d: kernel %.1f ms (first includes schedule build %.1f s)" %
+            (w, ms.value, time.perf_counter() - t_build))
+    kernel_ms = []
+    check(L.avs_synchronize(h))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step()
+        check(L.avs_last_time_ms(h, ctypes.byref(ms)))  # waits for this step's kernel
+        kernel_ms.append(ms.value)
+        log("step %d: kernel %.1f ms" % (s, ms.value))
+    check(L.avs_synchronize(h))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    flops = (8.0 / 3.0) * float(n) ** 3
+    value = flops / (ms_per_step * 1e-3) / 1e12
+    kmean = sum(kernel_ms) / len(kernel_ms)
+    achieved = flops / (kmean * 1e-3) / 1e12
+    peak = PEAK_FP64_TFLOPS if dt == np.float64 else PEAK_FP32_TFLOPS
+
+    # size-independent check on the full-size result: ||band||_F == ||A||_F
+    band = np.zeros((b + 1, n), order="F")
+    check(L.avs_get_band(h, dptr(band), b + 1))
+    fro_err = abs(float(np.linalg.norm(band)) - fro) / fro
+    L.avs_destroy(h)
+
+    sigma_check = None
+    ref_path = os.path.join(ROOT, "tests", "golden", "sigma_gauss_n%d_s%d.npy" % (args.check_n, SEED))
+    if args.check_n and os.path.exists(ref_path):
+        a_chk = avsvd.gaussian_matrix(args.check_n, SEED, dtype=dt)
+        T = avsvd.TiledMatrix(a_chk, b, b, sched=o.sched, tree=o.tree, domain=args.domain)
+        t_s = time.perf_counter()
+        sig = T.svdvals()
+        t_s = time.perf_counter() - t_s
+        ref = np.load(ref_path)
+        err = float(np.max(np.abs(np.sort(sig)[::-1] - ref)) / ref[0])
+        sigma_check = {"n": args.check_n, "max_abs_err_over_sigma_max": err,
+                       "vs": "FP64 LAPACK dgesdd of the same seeded matrix", "svdvals_seconds": t_s}
+        log("sigma check n=%d: %.3e" % (args.check_n, err))
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_blockbidiag_n%d_%s_b%d.json" % (n, args.dtype, b))
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: oracle n=%d" % args.cpu_n)
+        cpu = cpu_baseline(args.cpu_n, b)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic: iid N(0,1) entries, Philox4x32-10 seed %d" % SEED,
+        "config": {"workload": "Blockbidiag! n=%d %s tile b=%d, Gaussian input resident in HBM" %
+                               (n, args.dtype.upper(), b),
+                   "n": n, "b": b, "sched": args.sched, "tree": args.tree, "domain": args.domain or "auto",
+                   "parallelism": "single GPU"},
+        "pct_of_peak": round(100.0 * value / peak, 2),
+        "seconds_per_reduction": round(ms_per_step / 1e3, 4),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "kernel": "persistent_kernel" if args.sched == "persistent" else "host-enqueued launches",
+                     "kernel_ms": [round(x, 3) for x in kernel_ms]},
+        "band_fro_rel_err": fro_err,
+        "sigma_check": sigma_check,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * avsvd.h -- C ABI of the MI355X tiled SVD band-reduction engine (libavsvd.so).
+ *
+ * Drop-in boundary for the reference's stage-1 path (paths relative to the
+ * reference repository root):
+ *   TiledMatrix(A, b, b; tiledim2d)        KAbasedSVD/src/TiledMatrix.jl:41-67
+ *   tiled_QR!(A; kend)                     KAbasedSVD/src/tiledalgos.jl:6-15
+ *   Blockbidiag!(A; kend)                  KAbasedSVD/src/tiledalgos.jl:17-34
+ *   OOC_Bidiag!/OOC_SVD! (in-core branch)  KAbasedSVD/src/tiledalgos.jl:232-260
+ *   bidiagonalize / diag_lapack / my_CU_svdval
+ *                                          KAbasedSVD/src/bulgechasing.jl:33-53
+ *   tile kernels GEQRT/TSQRT/UNMQR/TSMQR   KAbasedSVD/src/QRkernels.jl:3-197
+ * Julia binds these with ccall (see avocado-sandwich_amd/julia/AvSVD.jl).
+ *
+ * Conventions
+ *  - Every function returns an avs_status: 0 = success, -i = argument i is
+ *    invalid, > 0 = runtime/device error (AVS_ERR_*).  No exceptions cross the
+ *    ABI.  avs_last_error() returns a message for the calling thread.
+ *  - Matrices are column-major with a leading dimension, like Julia arrays.
+ *  - A handle is bound to one HIP stream (avs_opts.stream, 0 = a stream the
+ *    handle creates).  Device work is asynchronous unless a function returns
+ *    host data.  Calls on one handle are not thread-safe; distinct handles are.
+ *  - Pointers named d* are device pointers, all others host pointers.
+ */
+#ifndef AVSVD_H
+#define AVSVD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t avs_status;
+
+enum {
+    AVS_OK = 0,
+    AVS_ERR_HIP = 1,        /* a HIP runtime call failed (message has the code) */
+    AVS_ERR_ALLOC = 2,      /* device or host allocation failed */
+    AVS_ERR_STATE = 3,      /* call out of order (e.g. no matrix set) */
+    AVS_ERR_LAPACK = 4,     /* host LAPACK not found or returned info != 0 */
+    AVS_ERR_UNSUPPORTED = 5,/* shape/tile/dtype combination not built */
+    AVS_ERR_BUDGET = 6      /* request exceeds the configured memory budget */
+};
+
+typedef enum { AVS_F64 = 0, AVS_F32 = 1 } avs_dtype;
+
+/* How the tile DAG is executed. */
+typedef enum {
+    AVS_SCHED_PERSISTENT = 0, /* one device-side dependency-counting persistent launch */
+    AVS_SCHED_HOST = 1        /* host-enqueued launches in the reference order (plain path) */
+} avs_sched;
+
+/* Reduction tree of each QR/LQ panel sweep.  FLAT is the reference's
+ * (TSQRT chain on the diagonal R, tiledalgos.jl:19-32); HIER splits the
+ * panel into domains of `domain` tiles, each a flat TS chain, whose R
+ * factors are then merged by triangle-on-triangle steps.  Both compute an
+ * orthogonal two-sided reduction with the same singular values. */
+typedef enum { AVS_TREE_FLAT = 0, AVS_TREE_HIER = 1 } avs_tree;
+
+/* Arithmetic of the FP32 trailing updates (FP64 handles ignore it). */
+typedef enum {
+    AVS_UPD_NATIVE = 0,     /* native FP32 MFMA, FP32 accumulate */
+    AVS_UPD_SPLIT_BF16 = 1  /* 3-term split-BF16 MFMA products, FP32 accumulate */
+} avs_update_mode;
+
+typedef struct avs_opts {
+    void *stream;          /* hipStream_t; 0 = create one */
+    int32_t sched;         /* avs_sched */
+    int32_t tree;          /* avs_tree */
+    int32_t domain;        /* tiles per domain for AVS_TREE_HIER (0 = auto) */
+    int32_t update_mode;   /* avs_update_mode */
+    int32_t workers;       /* persistent workgroups (0 = one per CU) */
+    int32_t reserved[9];   /* reserved[0]: bound in ms on any one dependency wait of
+                              the persistent executor (0 = 30000); exceeding it ends
+                              the launch and the next sync returns AVS_ERR_STATE */
+} avs_opts;
+
+typedef struct avs_handle avs_handle;
+
+/* Default options (persistent schedule, autotuned tree). */
+void avs_default_opts(avs_opts *opts);
+
+/* TiledMatrix(A, b, b) for an n x n matrix: allocates the tile-major working
+ * copy, tau/T factor storage and dependency counters.  Errors like the
+ * reference's "Not supported" (TiledMatrix.jl:42-44): n % b != 0 -> -3,
+ * unsupported b -> -4. */
+avs_status avs_create(avs_handle **h, int32_t dtype, int64_t n, int32_t b, const avs_opts *opts);
+avs_status avs_destroy(avs_handle *h);
+
+/* Load the matrix (element type = the handle's dtype), column-major, lda >= n. */
+avs_status avs_set_matrix_host(avs_handle *h, const void *A, int64_t lda);
+avs_status avs_set_matrix_device(avs_handle *h, const void *dA, int64_t lda);
+
+/* Read back the reduced n x n matrix in the reference's in-place layout:
+ * after Blockbidiag!, the band (upper-triangular diagonal tiles, lower-
+ * triangular super-diagonal tiles) and exact zeros elsewhere; after
+ * tiled_QR!, R in the upper triangle and the reflectors below it. */
+avs_status avs_get_matrix_host(avs_handle *h, void *A, int64_t lda);
+avs_status avs_get_matrix_device(avs_handle *h, void *dA, int64_t lda);
+
+/* Blockbidiag!(A; kend): two-sided QR/LQ reduction to upper band form of
+ * bandwidth b.  kend <= 0 means all n/b steps. */
+avs_status avs_blockbidiag(avs_handle *h, int32_t kend);
+/* tiled_QR!(A; kend). */
+avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
+
+/* Copy the band to the host in FP64 LAPACK upper-band layout:
+ * band[(b + i - j) + j*ldband] = A(i, j) for max(0, j-b) <= i <= j,
+ * ldband >= b+1.  Only (b+1) x n values cross PCIe, not n x n. */
+avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband);
+
+/* Full svdvals: band reduction (if not already run), band to host, block
+ * bulge chasing and dbdsdc in FP64 on the host.  sigma: n values, descending. */
+avs_status avs_svdvals(avs_handle *h, double *sigma);
+
+avs_status avs_synchronize(avs_handle *h);
+
+/* Device time of the last avs_blockbidiag/avs_tiled_qr, in milliseconds,
+ * measured with HIP events on the handle's stream (waits for completion). */
+avs_status avs_last_time_ms(avs_handle *h, double *ms);
+
+const char *avs_last_error(void);
+
+/* ---- host stage (kept on the host in FP64, as the reference runs it) ---- */
+
+/* bidiagonalize(A, bw), KAbasedSVD/src/bulgechasing.jl:33-42, on an upper
+ * band (layout as avs_get_band, bandwidth bw, ldband >= bw+1).  Outputs the
+ * diagonal d[n] and super-diagonal e[n-1] of the bidiagonal result. */
+avs_status avs_bidiagonalize_band(int64_t n, int32_t bw, const double *band, int64_t ldband,
+                                  double *d, double *e);
+/* diag_lapack, KAbasedSVD/src/bulgechasing.jl:50-53: dbdsdc('U','N').
+ * sigma[n] descending.  LAPACK is loaded at run time (AVSVD_LAPACK or the
+ * OpenBLAS bundled with scipy). */
+avs_status avs_bdsdc_values(int64_t n, const double *d, const double *e, double *sigma);
+
+/* ---- tile operator API (mirrors KAbasedSVD/src/QRkernels.jl kernels) ----
+ * One b x b tile per operand, element type dtype, device pointers with
+ * leading dimensions.  tau is a device array of b doubles.  `trans` runs the
+ * operator on the adjoint of every tile operand (the reference's LQ twins,
+ * TiledMatrix.jl:85-88).  stream may be 0 (the null stream). */
+/* QR_unsafe_kernel! (QRkernels.jl:3-66): in-place GEQRT, tau[0..b-2]. */
+avs_status avs_geqrt(int32_t dtype, int32_t b, void *dA, int64_t lda, double *dtau,
+                     int32_t trans, void *stream);
+/* QR_unsafe_kernel2! (QRkernels.jl:71-129): TSQRT of [triu(A1); A2]. */
+avs_status avs_tsqrt(int32_t dtype, int32_t b, void *dA1, int64_t lda1, void *dA2, int64_t lda2,
+                     double *dtau, int32_t trans, void *stream);
+/* applyQorQt_unsafe_kernel! (QRkernels.jl:131-161): C <- Q^T C (applyt) or
+ * Q C for the GEQRT factors in dV; C is b x ncols (ncols % b == 0).
+ * ctrans applies to C' (the reference's adjoint-view calls), vtrans reads V'. */
+avs_status avs_unmqr(int32_t dtype, int32_t b, const void *dV, int64_t ldv, const double *dtau,
+                     void *dC, int64_t ldc, int64_t ncols, int32_t applyt, int32_t vtrans,
+                     int32_t ctrans, void *stream);
+/* applyQorQt_unsafe_kernel2! (QRkernels.jl:165-197): [C1; C2] <- Q^T [C1; C2]
+ * (applyt) or Q [C1; C2] for the TSQRT reflectors in dV. */
+avs_status avs_tsmqr(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC2, int64_t ldc2,
+                     const void *dV, int64_t ldv, const double *dtau, int64_t ncols,
+                     int32_t applyt, int32_t vtrans, int32_t ctrans, void *stream);
+
+/* ---- seeded synthetic inputs (both sides of every comparison) ----
+ * Philox4x32-10 + Box-Muller, element e of the column-major matrix from
+ * counter e/2 and key = seed; entries N(0, scale^2). */
+avs_status avs_gen_gaussian_host(uint64_t seed, int64_t n, double scale, int32_t dtype, void *A,
+                                 int64_t lda);
+
+/* ---- diagnostics ----
+ * avs_debug_mfma_layout: runs one v_mfma_f64_16x16x4f64 and one
+ * v_mfma_f32_16x16x4f32 with A[i][k] = i + 16k + 1 and B[k][j] = ((k + 4j)*7 % 13) - 6
+ * and returns each lane's 4 accumulator values (out[lane*4 + m]), so a test
+ * can pin the lane layout the update kernels rely on.
+ * avs_schedule_stats: size of the task DAG and the list-scheduling makespan
+ * estimate (microseconds) for an nt x nt tile grid, without a device. */
+avs_status avs_debug_mfma_layout(double *out64, float *out32);
+avs_status avs_schedule_stats(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
+                              int32_t workers, int64_t *ntasks, double *est_us);
+/* The persistent executor's task list in dispatch order as a flat int32
+ * stream, per task: kind, mode, trans, ntiles, vtile, btile, c1tile, cstep,
+ * fac, ndep, ndep x (tile, count), nsig, nsig x tile.  alg: 0 Blockbidiag!,
+ * 1 tiled_QR!.  *needed gets the stream length; out is filled if cap allows.
+ * Lets a host test replay the DAG exactly as the device executes it. */
+avs_status avs_schedule_dump(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
+                             int32_t workers, int32_t *out, int64_t cap, int64_t *needed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AVSVD_H */

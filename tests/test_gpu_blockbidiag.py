@@ -1,0 +1,133 @@
+"""Blockbidiag! / tiled_QR! / svdvals on the GPU vs the oracle and LAPACK.
+
+Mirrors KAbasedSVD/test/test_tiledalgo.jl (sigma(band) = sigma(A), R of the
+tiled QR = LAPACK R up to signs) and CUDAbasedSVD/test.jl (svdvals through
+band + bulge chasing + bdsdc), for both executors (persistent DAG kernel and
+host-enqueued launches), both trees (the reference's flat tree and the
+hierarchical one), Float64 and Float32.  Bars: sigma error vs FP64 LAPACK
+<= 10*n*eps of the working precision; band structure exact.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import avsvd
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+BASE = 20261203
+
+
+def bar(dt, n):
+    return 10 * n * np.finfo(dt).eps
+
+
+def band_of(A, b):
+    n = A.shape[0]
+    out = np.zeros_like(A)
+    for d in range(b + 1):
+        idx = np.arange(n - d)
+        out[idx, idx + d] = A[idx, idx + d]
+    return out
+
+
+CASES = [
+    # (n, b, dtype, sched, tree, domain)
+    (64, 32, np.float64, avsvd.SCHED_HOST, avsvd.TREE_FLAT, 0),
+    (256, 32, np.float64, avsvd.SCHED_PERSISTENT, avsvd.TREE_FLAT, 0),
+    (256, 32, np.float64, avsvd.SCHED_PERSISTENT, avsvd.TREE_HIER, 2),
+    (256, 64, np.float64, avsvd.SCHED_HOST, avsvd.TREE_HIER, 2),
+    (512, 64, np.float64, avsvd.SCHED_PERSISTENT, avsvd.TREE_HIER, 3),
+    (512, 128, np.float64, avsvd.SCHED_PERSISTENT, avsvd.TREE_FLAT, 0),
+    (1024, 128, np.float64, avsvd.SCHED_PERSISTENT, avsvd.TREE_HIER, 2),
+    (256, 32, np.float32, avsvd.SCHED_PERSISTENT, avsvd.TREE_HIER, 2),
+    (512, 64, np.float32, avsvd.SCHED_HOST, avsvd.TREE_FLAT, 0),
+    (1024, 128, np.float32, avsvd.SCHED_PERSISTENT, avsvd.TREE_HIER, 2),
+]
+
+
+@pytest.mark.parametrize("n,b,dt,sched,tree,domain", CASES)
+def test_blockbidiag_sigma(gpu, golden_dir, n, b, dt, sched, tree, domain):
+    a64 = O.gaussian_matrix(n, BASE + n)
+    A = np.asfortranarray(a64.astype(dt))
+    print("case n=%d b=%d %s sched=%d tree=%d domain=%d" % (n, b, np.dtype(dt).name, sched, tree, domain),
+          flush=True)
+    T = avsvd.TiledMatrix(A, b, b, sched=sched, tree=tree, domain=domain, wait_ms=20000)
+    avsvd.Blockbidiag_(T)
+    out = T.A.astype(np.float64)
+    # exact zeros outside the band (the reference zeroes V tiles)
+    np.testing.assert_array_equal(out, band_of(out, b))
+    ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["gauss_n%d" % n] if n <= 1024 else sl.svdvals(a64)
+    err = O.sigma_err(sl.svdvals(out), ref)
+    print("sigma err vs FP64 LAPACK: %.3e (bar %.3e)" % (err, bar(dt, n)), flush=True)
+    assert err <= bar(dt, n)
+    # the band the device hands to the host stage equals the band of A
+    bnd = T.band()
+    for d in range(b + 1):
+        np.testing.assert_array_equal(bnd[b - d, d:], np.diagonal(out, d))
+
+
+@pytest.mark.parametrize("n,b", [(256, 32), (512, 128)])
+def test_flat_tree_band_matches_oracle(gpu, n, b):
+    """The reference schedule on the device gives the reference's band (same
+    reflectors; only the compact-WY arithmetic order differs)."""
+    a = O.gaussian_matrix(n, BASE + 3 * n)
+    ref = O.blockbidiag(a, b)
+    for sched in (avsvd.SCHED_HOST, avsvd.SCHED_PERSISTENT):
+        A = np.asfortranarray(a.copy())
+        T = avsvd.TiledMatrix(A, b, b, sched=sched, tree=avsvd.TREE_FLAT, wait_ms=20000)
+        avsvd.Blockbidiag_(T)
+        np.testing.assert_allclose(T.A, ref, rtol=0, atol=1e-11 * np.abs(a).max() * np.sqrt(n))
+
+
+@pytest.mark.parametrize("sched,tree", [(avsvd.SCHED_HOST, avsvd.TREE_FLAT),
+                                        (avsvd.SCHED_PERSISTENT, avsvd.TREE_HIER)])
+def test_tiled_qr_R(gpu, sched, tree):
+    n, b = 384, 64
+    a = O.gaussian_matrix(n, BASE + 9 * n)
+    A = np.asfortranarray(a.copy())
+    T = avsvd.TiledMatrix(A, b, b, sched=sched, tree=tree, domain=2, wait_ms=20000)
+    avsvd.tiled_QR_(T)
+    r = sl.qr(a, mode="r")[0]
+    out = np.triu(T.A)
+    np.testing.assert_allclose(out * np.sign(np.diag(out))[:, None], r * np.sign(np.diag(r))[:, None], atol=1e-10)
+
+
+def test_kend_partial_matches_oracle(gpu):
+    n, b, kend = 256, 32, 3
+    a = O.gaussian_matrix(n, BASE + 77)
+    ref = O.blockbidiag(a, b, kend=kend)
+    A = np.asfortranarray(a.copy())
+    T = avsvd.TiledMatrix(A, b, b, sched=avsvd.SCHED_PERSISTENT, tree=avsvd.TREE_FLAT, wait_ms=20000)
+    avsvd.Blockbidiag_(T, kend=kend)
+    np.testing.assert_allclose(T.A, ref, atol=1e-11 * np.sqrt(n))
+
+
+def test_config1_svdvals_end_to_end(gpu, golden_dir):
+    """BASELINE config 1 on the device path: n=1024, b=64, FP64."""
+    n, b = 1024, 64
+    a = np.asfortranarray(O.gaussian_matrix(n, BASE + 1))
+    ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["config1_n1024"]
+    sig = avsvd.svdvals(a, tilesize=b)
+    err = O.sigma_err(sig, ref)
+    print("config1 sigma err %.3e" % err, flush=True)
+    assert err <= 10 * n * np.finfo(np.float64).eps
+
+
+def test_n4096_fp64_b128_against_cached_gesdd(gpu, golden_dir):
+    """Full-size property checks: ||band||_F = ||A||_F and sigma vs the cached
+    FP64 dgesdd vector of the same seeded matrix."""
+    n, b = 4096, 128
+    a = avsvd.gaussian_matrix(n, BASE)
+    fro = np.linalg.norm(a)
+    T = avsvd.TiledMatrix(a, b, b, wait_ms=60000)
+    avsvd.Blockbidiag_(T, sync=False)
+    bnd = T.band()
+    assert abs(np.linalg.norm(bnd) - fro) <= 1e-12 * fro
+    sig = T.svdvals()
+    ref = np.load(os.path.join(golden_dir, "sigma_gauss_n4096_s%d.npy" % BASE))
+    err = O.sigma_err(sig, ref)
+    print("n=4096 sigma err %.3e" % err, flush=True)
+    assert err <= 1e-12

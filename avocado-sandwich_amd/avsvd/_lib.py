@@ -24,6 +24,8 @@ EXPORTS = (
     "avs_svdvals", "avs_synchronize", "avs_last_time_ms", "avs_last_error", "avs_bidiagonalize_band",
     "avs_bdsdc_values", "avs_geqrt", "avs_tsqrt", "avs_unmqr", "avs_tsmqr", "avs_gen_gaussian_host",
     "avs_debug_mfma_layout", "avs_schedule_stats", "avs_schedule_dump",
+    "avs_batch_create", "avs_batch_destroy", "avs_batch_set_matrix_host", "avs_batch_blockbidiag",
+    "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals",
 )
 
 
@@ -76,6 +78,13 @@ def lib():
             "avs_schedule_stats": [i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(i64), dp],
             "avs_schedule_dump": [i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int32), i64,
                                   ctypes.POINTER(i64)],
+            "avs_batch_create": [ctypes.POINTER(vp), i32, i32, ctypes.POINTER(i64), i32, ctypes.POINTER(Opts)],
+            "avs_batch_destroy": [vp],
+            "avs_batch_set_matrix_host": [vp, i32, vp, i64],
+            "avs_batch_blockbidiag": [vp],
+            "avs_batch_last_time_ms": [vp, dp],
+            "avs_batch_get_band": [vp, i32, dp, i64],
+            "avs_batch_svdvals": [vp, dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)

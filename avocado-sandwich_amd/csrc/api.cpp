@@ -7,7 +7,11 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <atomic>
+#include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "avsvd.h"
@@ -570,6 +574,261 @@ avs_status avs_schedule_dump(int32_t nt, int32_t b, int32_t alg, int32_t kend, i
     }
     *needed = (int64_t)v.size();
     if (out && cap >= (int64_t)v.size()) memcpy(out, v.data(), sizeof(int32_t) * v.size());
+    return AVS_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------- batched form
+struct avs_batch {
+    int dtype = AVS_F64, b = 0, count = 0;
+    std::vector<int64_t> n;                                   // true sizes
+    std::vector<int> nt;                                      // tiles per side (padded)
+    std::vector<long> off, soff, tile0, col0, boff;           // element / tile / column offsets
+    long total_tiles = 0, total_cols = 0, arena_elems = 0, band_elems = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    void *dArena = nullptr, *dStage = nullptr;
+    double *dBands = nullptr, *dScratch = nullptr;
+    long *dOff = nullptr, *dSoff = nullptr, *dTile0 = nullptr, *dCol0 = nullptr, *dBoff = nullptr;
+    int *dNts = nullptr, *dOrder = nullptr, *dNext = nullptr;
+    int workers = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool reduced = false, timed = false;
+    std::vector<double> hBands;
+};
+
+extern "C" {
+
+avs_status avs_batch_create(avs_batch **bp, int32_t dtype, int32_t count, const int64_t *ns, int32_t b,
+                            const avs_opts *opts)
+{
+    if (!bp) return fail(-1, "null batch pointer");
+    *bp = nullptr;
+    if (dtype != AVS_F64 && dtype != AVS_F32) return fail(-2, "dtype %d not supported", dtype);
+    if (count <= 0) return fail(-3, "count must be positive");
+    if (!ns) return fail(-4, "null sizes");
+    if (!supported_b(b)) return fail(-5, "Not supported: tile size %d (built: 32, 64, 128)", b);
+    std::unique_ptr<avs_batch> B(new avs_batch());
+    B->dtype = dtype;
+    B->b = b;
+    B->count = count;
+    B->n.assign(ns, ns + count);
+    long toff = 0, soffv = 0, t0 = 0, c0 = 0, bo = 0;
+    for (int i = 0; i < count; ++i) {
+        if (ns[i] <= 0) return fail(-4, "matrix %d has size %lld", i, (long long)ns[i]);
+        const int ntm = (int)((ns[i] + b - 1) / b);
+        const long N = (long)ntm * b;
+        B->nt.push_back(ntm);
+        B->off.push_back(toff);
+        B->soff.push_back(soffv);
+        B->tile0.push_back(t0);
+        B->col0.push_back(c0);
+        B->boff.push_back(bo);
+        toff += N * N;
+        soffv += N * N;
+        t0 += (long)ntm * ntm;
+        c0 += N;
+        bo += (long)(b + 1) * N;
+    }
+    B->arena_elems = toff;
+    B->total_tiles = t0;
+    B->total_cols = c0;
+    B->band_elems = bo;
+    if (opts && opts->stream) B->stream = (hipStream_t)opts->stream;
+    else {
+        HIPCHK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+        B->own_stream = true;
+    }
+    const size_t es = esize(dtype);
+    HIPCHK(hipMalloc(&B->dArena, es * (size_t)toff));
+    HIPCHK(hipMalloc(&B->dStage, es * (size_t)soffv));
+    HIPCHK(hipMemsetAsync(B->dStage, 0, es * (size_t)soffv, B->stream));
+    HIPCHK(hipMalloc(&B->dBands, sizeof(double) * (size_t)bo));
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return B->nt[x] > B->nt[y]; });
+    auto up = [&](void **d, const void *h, size_t bytes) -> avs_status {
+        HIPCHK(hipMalloc(d, bytes));
+        HIPCHK(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+        return AVS_OK;
+    };
+    avs_status st;
+    if ((st = up((void **)&B->dOff, B->off.data(), sizeof(long) * count))) return st;
+    if ((st = up((void **)&B->dSoff, B->soff.data(), sizeof(long) * count))) return st;
+    if ((st = up((void **)&B->dTile0, B->tile0.data(), sizeof(long) * count))) return st;
+    if ((st = up((void **)&B->dCol0, B->col0.data(), sizeof(long) * count))) return st;
+    if ((st = up((void **)&B->dBoff, B->boff.data(), sizeof(long) * count))) return st;
+    if ((st = up((void **)&B->dNts, B->nt.data(), sizeof(int) * count))) return st;
+    if ((st = up((void **)&B->dOrder, order.data(), sizeof(int) * count))) return st;
+    HIPCHK(hipMalloc(&B->dNext, sizeof(int)));
+    int dev = 0, ncu = 0, occ = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    st = dispatch_tb(dtype, b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int BB = decltype(bc)::value;
+        occ = batch_occupancy<TS, BB>();
+        return AVS_OK;
+    });
+    if (st) return st;
+    if (occ < 1) return fail(AVS_ERR_HIP, "batch kernel does not fit on a CU");
+    B->workers = std::min(ncu * occ, count);
+    if (opts && opts->workers > 0) B->workers = std::min(B->workers, (int)opts->workers);
+    HIPCHK(hipMalloc(&B->dScratch, sizeof(double) * 17 * b * (size_t)B->workers));
+    HIPCHK(hipEventCreate(&B->ev0));
+    HIPCHK(hipEventCreate(&B->ev1));
+    *bp = B.release();
+    return AVS_OK;
+}
+
+avs_status avs_batch_destroy(avs_batch *B)
+{
+    if (!B) return AVS_OK;
+    if (B->stream) hipStreamSynchronize(B->stream);
+    void *ptrs[] = {B->dArena, B->dStage, B->dBands, B->dScratch, B->dOff, B->dSoff, B->dTile0,
+                    B->dCol0, B->dBoff, B->dNts, B->dOrder, B->dNext};
+    for (void *p : ptrs) hipFree(p);
+    if (B->ev0) hipEventDestroy(B->ev0);
+    if (B->ev1) hipEventDestroy(B->ev1);
+    if (B->own_stream) hipStreamDestroy(B->stream);
+    delete B;
+    return AVS_OK;
+}
+
+avs_status avs_batch_set_matrix_host(avs_batch *B, int32_t idx, const void *A, int64_t lda)
+{
+    if (!B) return fail(-1, "null batch");
+    if (idx < 0 || idx >= B->count) return fail(-2, "matrix index %d out of range", idx);
+    if (!A) return fail(-3, "null matrix");
+    const int64_t n = B->n[idx];
+    if (lda < n) return fail(-4, "lda < n");
+    const size_t es = esize(B->dtype);
+    const long N = (long)B->nt[idx] * B->b;
+    char *dst = (char *)B->dStage + es * (size_t)B->soff[idx];
+    HIPCHK(hipMemcpy2DAsync(dst, es * N, A, es * lda, es * n, n, hipMemcpyHostToDevice, B->stream));
+    B->reduced = false;
+    return AVS_OK;
+}
+
+avs_status avs_batch_blockbidiag(avs_batch *B)
+{
+    if (!B) return fail(-1, "null batch");
+    avs_status st = dispatch_tb(B->dtype, B->b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int BB = decltype(bc)::value;
+        launch_batch_pack<TS>(B->dStage, B->dSoff, B->dArena, B->dOff, B->dNts, B->dTile0, B->total_tiles, B->count,
+                              BB, B->stream);
+        HIPCHK(hipMemsetAsync(B->dNext, 0, sizeof(int), B->stream));
+        HIPCHK(hipEventRecord(B->ev0, B->stream));
+        launch_batch<TS, BB>(B->workers, B->dArena, B->dOff, B->dNts, B->dOrder, B->count, B->dScratch, B->dNext,
+                             B->stream);
+        HIPCHK(hipEventRecord(B->ev1, B->stream));
+        HIPCHK(hipGetLastError());
+        return AVS_OK;
+    });
+    if (st) return st;
+    B->reduced = true;
+    B->timed = true;
+    return AVS_OK;
+}
+
+avs_status avs_batch_last_time_ms(avs_batch *B, double *ms)
+{
+    if (!B) return fail(-1, "null batch");
+    if (!ms) return fail(-2, "null output");
+    if (!B->timed) return fail(AVS_ERR_STATE, "nothing timed yet");
+    HIPCHK(hipEventSynchronize(B->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, B->ev0, B->ev1));
+    *ms = f;
+    return AVS_OK;
+}
+
+static avs_status batch_fetch_bands(avs_batch *B)
+{
+    avs_status st = dispatch_tb(B->dtype, B->b, [&](auto tp, auto) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        launch_batch_band<TS>(B->dArena, B->dOff, B->dNts, B->dCol0, B->total_cols, B->dBands, B->dBoff, B->count,
+                              B->b, B->stream);
+        HIPCHK(hipGetLastError());
+        return AVS_OK;
+    });
+    if (st) return st;
+    B->hBands.resize((size_t)B->band_elems);
+    HIPCHK(hipMemcpyAsync(B->hBands.data(), B->dBands, sizeof(double) * B->band_elems, hipMemcpyDeviceToHost,
+                          B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    return AVS_OK;
+}
+
+avs_status avs_batch_get_band(avs_batch *B, int32_t idx, double *band, int64_t ldband)
+{
+    if (!B) return fail(-1, "null batch");
+    if (idx < 0 || idx >= B->count) return fail(-2, "matrix index %d out of range", idx);
+    if (!band) return fail(-3, "null band");
+    if (ldband < B->b + 1) return fail(-4, "ldband < b+1");
+    if (!B->reduced) return fail(AVS_ERR_STATE, "batch not reduced yet");
+    if (B->hBands.empty()) {
+        avs_status st = batch_fetch_bands(B);
+        if (st) return st;
+    }
+    const long N = (long)B->nt[idx] * B->b;
+    const double *src = B->hBands.data() + B->boff[idx];
+    for (long j = 0; j < N; ++j) memcpy(band + j * ldband, src + j * (B->b + 1), sizeof(double) * (B->b + 1));
+    return AVS_OK;
+}
+
+avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
+{
+    if (!B) return fail(-1, "null batch");
+    if (!sigma) return fail(-2, "null sigma");
+    avs_status st;
+    if (!B->reduced) {
+        if ((st = avs_batch_blockbidiag(B))) return st;
+    }
+    B->hBands.clear();
+    if ((st = batch_fetch_bands(B))) return st;
+    std::vector<long> soff(B->count, 0);
+    for (int i = 1; i < B->count; ++i) soff[i] = soff[i - 1] + B->n[i - 1];
+    std::vector<int> order(B->count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return B->nt[x] > B->nt[y]; });
+    std::atomic<int> next{0};
+    std::atomic<int> first_err{0};
+    std::string err_msg;
+    std::mutex mu;
+    auto work = [&]() {
+        std::vector<double> d, e, sg;
+        for (;;) {
+            const int q = next.fetch_add(1);
+            if (q >= B->count) break;
+            const int m = order[q];
+            const long N = (long)B->nt[m] * B->b;
+            d.resize(N);
+            e.resize(std::max<long>(N - 1, 1));
+            sg.resize(N);
+            avs_status s2 = bidiagonalize_band_threads(N, B->b, B->hBands.data() + B->boff[m], B->b + 1, d.data(),
+                                                       e.data(), 1);
+            if (!s2) s2 = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
+            if (s2) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!first_err) {
+                    first_err = s2;
+                    err_msg = g_err;
+                }
+                continue;
+            }
+            // padding adds N - n exact zero singular values: keep the n largest
+            memcpy(sigma + soff[m], sg.data(), sizeof(double) * B->n[m]);
+        }
+    };
+    const int nth = std::max(1, std::min(host_thread_count(), B->count));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+    if (first_err) return fail(first_err, "%s", err_msg.c_str());
     return AVS_OK;
 }
 

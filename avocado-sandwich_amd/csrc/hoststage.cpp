@@ -14,7 +14,10 @@
 #include <glob.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
+#include <thread>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -42,78 +45,169 @@ struct Band {
 
 inline double sgn1(double x) { return x >= 0.0 ? 1.0 : -1.0; }
 
-// Householder QR of the m x nc column-major block B, keeping R (zeros below).
-void qr_keep_r(long m, long nc, double *B, long ld)
+// Householder QR of a wide m x nc block stored ROW-major (row r at B + r*ld),
+// keeping R (zeros below).  Same arithmetic, in the same order, as a
+// column-major unblocked QR: w_c = B[j][c] + sum_{r>j} v_r B[r][c] accumulated
+// over r ascending, then B[r][c] -= (tau w_c) v_r.  Row-major makes both inner
+// loops run over contiguous c, so they vectorise without reassociation.
+void qr_keep_r_rows(long m, long nc, double *B, long ld, double *w)
 {
     const long kmax = std::min(m, nc);
     for (long j = 0; j < kmax; ++j) {
-        double *cj = B + j * ld;
         double s = 0.0;
-        for (long r = j + 1; r < m; ++r) s += cj[r] * cj[r];
-        const double alpha = cj[j];
+        for (long r = j + 1; r < m; ++r) s += B[r * ld + j] * B[r * ld + j];
+        const double alpha = B[j * ld + j];
         const double nrm = std::sqrt(s + alpha * alpha);
         if (nrm == 0.0) continue;
         const double beta = -sgn1(alpha) * nrm;
         const double tau = (beta - alpha) / beta;
         const double scal = 1.0 / (alpha - beta);
-        for (long r = j + 1; r < m; ++r) cj[r] *= scal;
-        for (long c = j + 1; c < nc; ++c) {
-            double *cc = B + c * ld;
-            double w = cc[j];
-            for (long r = j + 1; r < m; ++r) w += cj[r] * cc[r];
-            w *= tau;
-            cc[j] -= w;
-            for (long r = j + 1; r < m; ++r) cc[r] -= w * cj[r];
+        for (long r = j + 1; r < m; ++r) B[r * ld + j] *= scal;
+        const long c0 = j + 1, nw = nc - c0;
+        if (nw > 0) {
+            const double *bj = B + j * ld + c0;
+            for (long c = 0; c < nw; ++c) w[c] = bj[c];
+            for (long r = j + 1; r < m; ++r) {
+                const double v = B[r * ld + j];
+                const double *br = B + r * ld + c0;
+                for (long c = 0; c < nw; ++c) w[c] += v * br[c];
+            }
+            for (long c = 0; c < nw; ++c) w[c] *= tau;
+            double *bjw = B + j * ld + c0;
+            for (long c = 0; c < nw; ++c) bjw[c] -= w[c];
+            for (long r = j + 1; r < m; ++r) {
+                const double v = B[r * ld + j];
+                double *br = B + r * ld + c0;
+                for (long c = 0; c < nw; ++c) br[c] -= w[c] * v;
+            }
         }
-        cj[j] = beta;
-        for (long r = j + 1; r < m; ++r) cj[r] = 0.0;
+        B[j * ld + j] = beta;
+        for (long r = j + 1; r < m; ++r) B[r * ld + j] = 0.0;
     }
 }
+
+struct Scratch {
+    std::vector<double> buf, w;
+};
 
 // QR_col!(A, s, l, gap) (1-based, bulgechasing.jl:9-13): rows s..l, columns
 // s..l+gap replaced by R.
-void qr_col(Band &A, std::vector<double> &buf, long s, long l, long gap)
+void qr_col(Band &A, Scratch &S, long s, long l, long gap)
 {
     const long m = l - s + 1, nc = l + gap - s + 1;
     if (m <= 0 || nc <= 0) return;
-    buf.assign((size_t)(m * nc), 0.0);
+    S.buf.assign((size_t)(m * nc), 0.0);
+    S.w.resize((size_t)nc);
     const long r0 = s - 1, c0 = s - 1;
-    for (long c = 0; c < nc; ++c)
-        for (long r = 0; r < m; ++r) buf[r + c * m] = A.get(r0 + r, c0 + c);
-    qr_keep_r(m, nc, buf.data(), m);
-    for (long c = 0; c < nc; ++c)
-        for (long r = 0; r < m; ++r) A.set(r0 + r, c0 + c, buf[r + c * m]);
+    for (long r = 0; r < m; ++r)
+        for (long c = 0; c < nc; ++c) S.buf[r * nc + c] = A.get(r0 + r, c0 + c);
+    qr_keep_r_rows(m, nc, S.buf.data(), nc, S.w.data()); // m <= nc since gap >= 0
+    for (long r = 0; r < m; ++r)
+        for (long c = 0; c < nc; ++c) A.set(r0 + r, c0 + c, S.buf[r * nc + c]);
 }
 
 // QR_row!(A, s, l, gap) (bulgechasing.jl:1-7): rows s..l, columns s+gap..l
-// replaced by L of their LQ factorisation.
-void qr_row(Band &A, std::vector<double> &buf, long s, long l, long gap)
+// replaced by L of their LQ factorisation (QR of the transposed block).
+void qr_row(Band &A, Scratch &S, long s, long l, long gap)
 {
     const long m = l - s + 1, nc = l - s - gap + 1;
     if (m <= 0 || nc <= 0) return;
-    buf.assign((size_t)(m * nc), 0.0); // transposed block: nc x m
+    // the transposed block (nc x m) stored row-major is the block column-major:
+    // row c of the transpose = column c of the block
+    S.buf.assign((size_t)(m * nc), 0.0);
+    S.w.resize((size_t)m);
     const long r0 = s - 1, c0 = s - 1 + gap;
-    for (long r = 0; r < m; ++r)
-        for (long c = 0; c < nc; ++c) buf[c + r * nc] = A.get(r0 + r, c0 + c);
-    qr_keep_r(nc, m, buf.data(), nc);
-    for (long r = 0; r < m; ++r)
-        for (long c = 0; c < nc; ++c) A.set(r0 + r, c0 + c, buf[c + r * nc]);
+    for (long c = 0; c < nc; ++c)
+        for (long r = 0; r < m; ++r) S.buf[c * m + r] = A.get(r0 + r, c0 + c);
+    qr_keep_r_rows(nc, m, S.buf.data(), m, S.w.data()); // nc <= m since gap >= 0
+    for (long c = 0; c < nc; ++c)
+        for (long r = 0; r < m; ++r) A.set(r0 + r, c0 + c, S.buf[c * m + r]);
 }
 
-// block_bidiagonalize!(A, n, bandwidth, target_bandwidth), bulgechasing.jl:17-31
-void block_bidiagonalize(Band &A, long n, long bw, long tbw)
+// One block operation of block_bidiagonalize!; its block spans rows [s, l].
+struct BOp {
+    int kind; // 0 = QR_row, 1 = QR_col
+    long s, l, gap;
+};
+
+// ops of sweep j of block_bidiagonalize!(A, n, bw, tbw), bulgechasing.jl:18-29
+void sweep_ops(long n, long bw, long tbw, long j, std::vector<BOp> &ops)
 {
-    std::vector<double> buf;
-    for (long j = 1; j <= n - tbw; j += tbw) {
-        qr_row(A, buf, j, std::min(j + bw + tbw - 1, n), tbw);
-        for (long i = j; i <= n; i += bw) {
-            const long last = std::min(i + bw + tbw - 1, n);
-            const long sc = std::min(bw + tbw - 1, std::max(n - i - bw - tbw + 1, 0L));
-            qr_col(A, buf, i + tbw, last, sc);
-            if (i + tbw > n - bw) break;
-            qr_row(A, buf, i + tbw, last + sc, bw);
-        }
+    ops.clear();
+    ops.push_back({0, j, std::min(j + bw + tbw - 1, n), tbw});
+    for (long i = j; i <= n; i += bw) {
+        const long last = std::min(i + bw + tbw - 1, n);
+        const long sc = std::min(bw + tbw - 1, std::max(n - i - bw - tbw + 1, 0L));
+        ops.push_back({1, i + tbw, last, sc});
+        if (i + tbw > n - bw) break;
+        ops.push_back({0, i + tbw, last + sc, bw});
     }
+}
+
+void run_op(Band &A, Scratch &S, const BOp &o)
+{
+    if (o.kind == 0) qr_row(A, S, o.s, o.l, o.gap);
+    else qr_col(A, S, o.s, o.l, o.gap);
+}
+
+int host_threads()
+{
+    if (const char *e = getenv("AVSVD_HOST_THREADS")) {
+        const int v = atoi(e);
+        if (v > 0) return v;
+    }
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, hc));
+}
+
+// block_bidiagonalize!(A, n, bandwidth, target_bandwidth), bulgechasing.jl:17-31.
+// Sweep j+tbw trails sweep j: before an operation whose block spans rows
+// [s, l], a sweep waits until the previous sweep's next operation starts below
+// row l.  Rows of a sweep's operations only move down, so every pair of
+// operations with overlapping blocks runs in the sequential order and the
+// result is bit-identical to the sequential loop.
+void block_bidiagonalize(Band &A, long n, long bw, long tbw, int nthreads)
+{
+    std::vector<long> starts;
+    for (long j = 1; j <= n - tbw; j += tbw) starts.push_back(j);
+    const long ns = (long)starts.size();
+    if (ns == 0) return;
+    if (nthreads <= 1 || ns < 4) {
+        Scratch S;
+        std::vector<BOp> ops;
+        for (long j : starts) {
+            sweep_ops(n, bw, tbw, j, ops);
+            for (auto &o : ops) run_op(A, S, o);
+        }
+        return;
+    }
+    const long DONE = LONG_MAX;
+    std::vector<std::atomic<long>> front(ns);
+    for (auto &f : front) f.store(LONG_MIN, std::memory_order_relaxed); // not started
+    auto worker = [&](int t) {
+        Scratch S;
+        std::vector<BOp> ops;
+        for (long w = t; w < ns; w += nthreads) {
+            sweep_ops(n, bw, tbw, starts[w], ops);
+            for (size_t k = 0; k < ops.size(); ++k) {
+                front[w].store(ops[k].s, std::memory_order_release);
+                if (w > 0) {
+                    const long need_below = ops[k].l; // previous sweep must be past row l
+                    while (true) {
+                        const long f = front[w - 1].load(std::memory_order_acquire);
+                        if (f == DONE || f > need_below) break;
+                        std::this_thread::yield();
+                    }
+                }
+                run_op(A, S, ops[k]);
+            }
+            front[w].store(DONE, std::memory_order_release);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nthreads; ++t) th.emplace_back(worker, t);
+    worker(0);
+    for (auto &x : th) x.join();
 }
 
 // ------------------------------------------------------------------ LAPACK
@@ -159,7 +253,16 @@ extern "C" {
 
 avs_status avs_bidiagonalize_band(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e)
 {
-    using avs::fail;
+    return avs::bidiagonalize_band_threads(n, bw, band, ldband, d, e, host_threads());
+}
+
+} // extern "C"
+
+namespace avs {
+
+avs_status bidiagonalize_band_threads(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,
+                                      int nth)
+{
     if (n <= 0) return fail(-1, "n must be positive");
     if (bw < 1) return fail(-2, "bandwidth must be >= 1");
     if (!band) return fail(-3, "null band");
@@ -174,15 +277,21 @@ avs_status avs_bidiagonalize_band(int64_t n, int32_t bw, const double *band, int
     long b = bw;
     while (b > 16) {
         const long half = (long)std::nearbyint(b / 2.0);
-        block_bidiagonalize(A, n, half * 2, half);
+        block_bidiagonalize(A, n, half * 2, half, nth);
         b = half;
     }
-    block_bidiagonalize(A, n, b, 1);
+    block_bidiagonalize(A, n, b, 1, nth);
     if (A.overflow) return fail(AVS_ERR_STATE, "bulge outside the band store (internal error)");
     for (long i = 0; i < n; ++i) d[i] = A.get(i, i);
     for (long i = 0; i + 1 < n; ++i) e[i] = A.get(i, i + 1);
     return AVS_OK;
 }
+
+int host_thread_count() { return host_threads(); }
+
+} // namespace avs
+
+extern "C" {
 
 avs_status avs_bdsdc_values(int64_t n, const double *d, const double *e, double *sigma)
 {

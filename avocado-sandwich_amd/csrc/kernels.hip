@@ -95,9 +95,11 @@ __global__ void __launch_bounds__(NTHREADS) update_kernel(const TS *V, long ldv,
 }
 
 // ------------------------------------------------------ persistent executor
-__device__ __forceinline__ int ld_acquire(const int *p)
+// relaxed agent-scope load for spinning (no cache invalidate per poll); the
+// acquire fence after the wait orders the data reads that follow
+__device__ __forceinline__ int ld_relaxed(const int *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename TS, int B>
@@ -123,13 +125,15 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
         if (tk >= ntasks) break;
         const Task T = tasks[tk];
         // wait for dependencies, bounded in wall-clock time so that a schedule
-        // error ends the launch with a status instead of hanging it
+        // error ends the launch with a status instead of hanging it.  Thread 0
+        // polls with relaxed loads, then one agent-scope acquire fence (cache
+        // invalidate) orders the workgroup's reads after the producers' writes.
         if (t == 0) {
             const long long t0 = wall_clock64();
             for (int d = 0; d < T.ndep; ++d) {
                 const int *p = cnt + dep_tile[T.dep0 + d];
                 const int need = dep_val[T.dep0 + d];
-                while (ld_acquire(p) < need) {
+                while (ld_relaxed(p) < need) {
                     __builtin_amdgcn_s_sleep(1);
                     if (wall_clock64() - t0 > wait_ticks) {
                         sh_fail = 1;
@@ -138,13 +142,13 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
                 }
                 if (sh_fail) break;
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (sh_fail) {
             if (t == 0) atomicExch(status, 1 + tk);
             break;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         double *tau = taus + (long)T.fac * B;
         double *tjj = tjjs + (long)T.fac * 16 * B;
         const bool tr = T.trans != 0;
@@ -166,13 +170,162 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
             else
                 update_body<TS, B, MODE_TT, true>(V, B, tr, tjj, C1, B, st, C2, B, st, tr, T.ntiles, smem);
         }
+        __builtin_amdgcn_s_waitcnt(0); // this thread's global stores are acknowledged
         __syncthreads();
+        // every thread's stores are acknowledged before the barrier; thread 0's
+        // agent-scope release fence (L2 write-back) then publishes them before
+        // the counters move
         if (t == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             for (int s = 0; s < T.nsig; ++s)
-                __hip_atomic_fetch_add(cnt + sig_tile[T.sig0 + s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(cnt + sig_tile[T.sig0 + s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+// ---------------------------------------------------------- batched form
+// One grouped persistent launch for many independent matrices (possibly of
+// different sizes, all with tile size B).  Each workgroup takes whole matrices
+// from a queue (largest first) and runs the reference's flat-tree
+// Blockbidiag! schedule on it (tiledalgos.jl:17-34) with the same tile
+// operators as the single-matrix path; all reflector factors of the matrix
+// being reduced live in a per-workgroup scratch slot.
+template <typename TS, int B>
+__global__ void __launch_bounds__(NTHREADS, 1) batch_kernel(TS *arena, const long *__restrict__ off,
+                                                             const int *__restrict__ nts,
+                                                             const int *__restrict__ order, int count,
+                                                             double *scratch, int *next)
+{
+    __shared__ double smem[Lds<TS, B>::maxv];
+    __shared__ int sh_m;
+    const int t = threadIdx.x;
+    const long tsz = (long)B * B;
+    double *tau = scratch + (long)blockIdx.x * 17 * B;
+    double *tjj = tau + B;
+    for (;;) {
+        if (t == 0) sh_m = atomicAdd(next, 1);
+        __syncthreads();
+        const int qi = sh_m;
+        if (qi >= count) break;
+        const int m = order[qi];
+        const int nt = nts[m];
+        TS *T = arena + off[m];
+        auto tile = [&](int i, int j) { return T + ((long)j * nt + i) * tsz; };
+        for (int k = 0; k < nt; ++k) {
+            const int nc = nt - k - 1;
+            // QR sweep on tile column k
+            panel_body<TS, B, MODE_GE>(tile(k, k), B, tile(k, k), B, false, tau, tjj, smem);
+            __syncthreads();
+            if (nc > 0)
+                update_body<TS, B, MODE_GE, true>(tile(k, k), B, false, tjj, tile(k, k + 1), B, nt * tsz, tile(k, k + 1),
+                                                   B, nt * tsz, false, nc, smem);
+            __syncthreads();
+            for (int i = k + 1; i < nt; ++i) {
+                panel_body<TS, B, MODE_TS>(tile(k, k), B, tile(i, k), B, false, tau, tjj, smem);
+                __syncthreads();
+                if (nc > 0)
+                    update_body<TS, B, MODE_TS, true>(tile(i, k), B, false, tjj, tile(k, k + 1), B, nt * tsz,
+                                                       tile(i, k + 1), B, nt * tsz, false, nc, smem);
+                __syncthreads();
+            }
+            if (k == nt - 1) break;
+            // LQ sweep on tile row k from column k+1 (adjoint addressing)
+            panel_body<TS, B, MODE_GE>(tile(k, k + 1), B, tile(k, k + 1), B, true, tau, tjj, smem);
+            __syncthreads();
+            update_body<TS, B, MODE_GE, true>(tile(k, k + 1), B, true, tjj, tile(k + 1, k + 1), B, tsz,
+                                               tile(k + 1, k + 1), B, tsz, true, nc, smem);
+            __syncthreads();
+            for (int j = k + 2; j < nt; ++j) {
+                panel_body<TS, B, MODE_TS>(tile(k, k + 1), B, tile(k, j), B, true, tau, tjj, smem);
+                __syncthreads();
+                update_body<TS, B, MODE_TS, true>(tile(k, j), B, true, tjj, tile(k + 1, k + 1), B, tsz, tile(k + 1, j),
+                                                   B, tsz, true, nc, smem);
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// column-major padded staging (one N_m x N_m block per matrix, N_m = nt_m*B)
+// -> tile-major arena; one block per tile of any matrix
+template <typename TS>
+__global__ void __launch_bounds__(256) batch_pack_kernel(const TS *__restrict__ stage, const long *__restrict__ soff,
+                                                         TS *__restrict__ arena, const long *__restrict__ off,
+                                                         const int *__restrict__ nts, const long *__restrict__ tile0,
+                                                         int count, int b)
+{
+    const long g = blockIdx.x;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) { // last matrix with tile0 <= g
+        const int mid = (lo + hi + 1) / 2;
+        if (tile0[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int m = lo, nt = nts[m];
+    const long lt = g - tile0[m];
+    const int i = (int)(lt % nt), j = (int)(lt / nt);
+    const long N = (long)nt * b;
+    const TS *src = stage + soff[m] + (long)i * b + (long)j * b * N;
+    TS *dst = arena + off[m] + lt * b * b;
+    for (int idx = threadIdx.x; idx < b * b; idx += blockDim.x) {
+        const int r = idx % b, c = idx / b;
+        dst[idx] = src[r + (long)c * N];
+    }
+}
+
+// bands of all matrices: band m at boff[m], (b+1) x N_m, LAPACK upper-band layout
+template <typename TS>
+__global__ void __launch_bounds__(256) batch_band_kernel(const TS *__restrict__ arena, const long *__restrict__ off,
+                                                         const int *__restrict__ nts, const long *__restrict__ col0,
+                                                         double *__restrict__ bands, const long *__restrict__ boff,
+                                                         int count, int b)
+{
+    const long g = blockIdx.x;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (col0[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int m = lo, nt = nts[m];
+    const long j = g - col0[m];
+    const TS *tiles = arena + off[m];
+    double *band = bands + boff[m];
+    for (int d = threadIdx.x; d <= b; d += blockDim.x) {
+        const long i = j - b + d;
+        double v = 0.0;
+        if (i >= 0) {
+            const long I = i / b, J = j / b;
+            v = (double)tiles[(J * nt + I) * b * b + (i % b) + (j % b) * (long)b];
+        }
+        band[d + j * (long)(b + 1)] = v;
+    }
+}
+
+template <typename TS, int B>
+void launch_batch(int workers, void *arena, const long *off, const int *nts, const int *order, int count,
+                  double *scratch, int *next, hipStream_t s)
+{
+    hipLaunchKernelGGL((batch_kernel<TS, B>), dim3(workers), dim3(NTHREADS), 0, s, (TS *)arena, off, nts, order, count,
+                       scratch, next);
+}
+template <typename TS, int B> int batch_occupancy()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<TS, B>, NTHREADS, 0) != hipSuccess) return 0;
+    return nb;
+}
+template <typename TS>
+void launch_batch_pack(const void *stage, const long *soff, void *arena, const long *off, const int *nts,
+                       const long *tile0, long total_tiles, int count, int b, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_pack_kernel<TS>, dim3((unsigned)total_tiles), dim3(256), 0, s, (const TS *)stage, soff,
+                       (TS *)arena, off, nts, tile0, count, b);
+}
+template <typename TS>
+void launch_batch_band(const void *arena, const long *off, const int *nts, const long *col0, long total_cols,
+                       double *bands, const long *boff, int count, int b, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_band_kernel<TS>, dim3((unsigned)total_cols), dim3(256), 0, s, (const TS *)arena, off, nts,
+                       col0, bands, boff, count, b);
 }
 
 // ------------------------------------------------------------ debug probes
@@ -292,7 +445,10 @@ void launch_mfma_probe(const double *a, const double *b, double *o64, const floa
                                        void *, long, long, int, int, int, hipStream_t);                                \
     template void launch_persistent<TS, B>(int, void *, double *, double *, const Task *, int, const int *,           \
                                            const int *, const int *, int *, int *, long long, int *, hipStream_t);    \
-    template int persistent_occupancy<TS, B>();
+    template int persistent_occupancy<TS, B>();                                                                        \
+    template void launch_batch<TS, B>(int, void *, const long *, const int *, const int *, int, double *, int *,      \
+                                      hipStream_t);                                                                    \
+    template int batch_occupancy<TS, B>();
 
 AVS_INST_B(double, 32)
 AVS_INST_B(double, 64)
@@ -306,6 +462,14 @@ template void launch_pack<float>(const void *, long, void *, int, int, hipStream
 template void launch_unpack<double>(const void *, void *, long, int, int, int, int, hipStream_t);
 template void launch_unpack<float>(const void *, void *, long, int, int, int, int, hipStream_t);
 template void launch_band<double>(const void *, double *, long, int, int, hipStream_t);
+template void launch_batch_pack<double>(const void *, const long *, void *, const long *, const int *, const long *,
+                                        long, int, int, hipStream_t);
+template void launch_batch_pack<float>(const void *, const long *, void *, const long *, const int *, const long *,
+                                       long, int, int, hipStream_t);
+template void launch_batch_band<double>(const void *, const long *, const int *, const long *, long, double *,
+                                        const long *, int, int, hipStream_t);
+template void launch_batch_band<float>(const void *, const long *, const int *, const long *, long, double *,
+                                       const long *, int, int, hipStream_t);
 template void launch_band<float>(const void *, double *, long, int, int, hipStream_t);
 
 } // namespace avs

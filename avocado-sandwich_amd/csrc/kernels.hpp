@@ -26,6 +26,17 @@ void launch_persistent(int workers, void *tiles, double *taus, double *tjjs, con
                        long long wait_ticks, int *status, hipStream_t s);
 template <typename TS, int B> int persistent_occupancy();
 
+template <typename TS, int B>
+void launch_batch(int workers, void *arena, const long *off, const int *nts, const int *order, int count,
+                  double *scratch, int *next, hipStream_t s);
+template <typename TS, int B> int batch_occupancy();
+template <typename TS>
+void launch_batch_pack(const void *stage, const long *soff, void *arena, const long *off, const int *nts,
+                       const long *tile0, long total_tiles, int count, int b, hipStream_t s);
+template <typename TS>
+void launch_batch_band(const void *arena, const long *off, const int *nts, const long *col0, long total_cols,
+                       double *bands, const long *boff, int count, int b, hipStream_t s);
+
 void launch_mfma_probe(const double *a, const double *b, double *o64, const float *af, const float *bf, float *o32,
                        hipStream_t s);
 

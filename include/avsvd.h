@@ -163,6 +163,30 @@ avs_status avs_tsmqr(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC
 avs_status avs_gen_gaussian_host(uint64_t seed, int64_t n, double scale, int32_t dtype, void *A,
                                  int64_t lda);
 
+/* ---- batched form ----
+ * Many independent square matrices of mixed size n_i (each zero-padded to a
+ * multiple of b; padding only adds zero singular values, which are dropped),
+ * band-reduced in one grouped persistent launch: every workgroup takes whole
+ * matrices from a queue, largest first, and runs the reference's Blockbidiag!
+ * schedule on them.  The host stage then runs on a thread pool, one matrix
+ * per thread.  No reference counterpart (the reference reduces one matrix per
+ * call); it serves the batched svdvals workload. */
+typedef struct avs_batch avs_batch;
+avs_status avs_batch_create(avs_batch **batch, int32_t dtype, int32_t count, const int64_t *ns, int32_t b,
+                            const avs_opts *opts);
+avs_status avs_batch_destroy(avs_batch *batch);
+/* Stage matrix idx (n_idx x n_idx, column-major, the batch dtype) for upload. */
+avs_status avs_batch_set_matrix_host(avs_batch *batch, int32_t idx, const void *A, int64_t lda);
+/* One grouped launch over all matrices (re-packs the staged inputs first). */
+avs_status avs_batch_blockbidiag(avs_batch *batch);
+avs_status avs_batch_last_time_ms(avs_batch *batch, double *ms);
+/* Band of matrix idx, FP64 upper-band layout, ldband >= b+1, N_idx columns
+ * (N_idx = n_idx rounded up to a multiple of b). */
+avs_status avs_batch_get_band(avs_batch *batch, int32_t idx, double *band, int64_t ldband);
+/* All singular values, concatenated in matrix order (n_i values each,
+ * descending); runs the reduction first if needed. */
+avs_status avs_batch_svdvals(avs_batch *batch, double *sigma);
+
 /* ---- diagnostics ----
  * avs_debug_mfma_layout: runs one v_mfma_f64_16x16x4f64 and one
  * v_mfma_f32_16x16x4f32 with A[i][k] = i + 16k + 1 and B[k][j] = ((k + 4j)*7 % 13) - 6

@@ -25,8 +25,16 @@ EXPORTS = (
     "avs_bdsdc_values", "avs_geqrt", "avs_tsqrt", "avs_unmqr", "avs_tsmqr", "avs_gen_gaussian_host",
     "avs_debug_mfma_layout", "avs_schedule_stats", "avs_schedule_dump",
     "avs_batch_create", "avs_batch_destroy", "avs_batch_set_matrix_host", "avs_batch_blockbidiag",
-    "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals",
+    "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals", "avs_set_matrix_host_sub",
+    "avs_server_default_opts", "avs_server_create", "avs_server_submit", "avs_server_poll", "avs_server_wait",
+    "avs_server_stats", "avs_server_destroy",
 )
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
+                ("host_threads", ctypes.c_int32), ("b_f32", ctypes.c_int32), ("b_f64", ctypes.c_int32),
+                ("update_mode", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
 
 
 class Opts(ctypes.Structure):
@@ -54,6 +62,8 @@ def lib():
         L.avs_last_error.restype = ctypes.c_char_p
         L.avs_default_opts.argtypes = [ctypes.POINTER(Opts)]
         L.avs_default_opts.restype = None
+        L.avs_server_default_opts.argtypes = [ctypes.POINTER(ServerOpts)]
+        L.avs_server_default_opts.restype = None
         sig = {
             "avs_create": [ctypes.POINTER(vp), i32, i64, i32, ctypes.POINTER(Opts)],
             "avs_destroy": [vp],
@@ -85,6 +95,13 @@ def lib():
             "avs_batch_last_time_ms": [vp, dp],
             "avs_batch_get_band": [vp, i32, dp, i64],
             "avs_batch_svdvals": [vp, dp],
+            "avs_set_matrix_host_sub": [vp, vp, i64, i64],
+            "avs_server_create": [ctypes.POINTER(vp), ctypes.POINTER(ServerOpts)],
+            "avs_server_submit": [vp, i32, i64, vp, i64, ctypes.POINTER(ctypes.c_uint64)],
+            "avs_server_poll": [vp, ctypes.c_uint64, ctypes.POINTER(i32)],
+            "avs_server_wait": [vp, ctypes.c_uint64, dp, dp],
+            "avs_server_stats": [vp, dp, dp, dp, ctypes.POINTER(i64)],
+            "avs_server_destroy": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)

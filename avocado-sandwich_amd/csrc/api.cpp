@@ -6,8 +6,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <tuple>
 #include <atomic>
 #include <numeric>
 #include <string>
@@ -67,16 +69,28 @@ struct DevProgram {
     Program P;
     Task *dTasks = nullptr;
     int *dDepTile = nullptr, *dDepVal = nullptr, *dSig = nullptr;
-    void release()
+    int device = -1;
+    ~DevProgram()
     {
         hipFree(dTasks);
         hipFree(dDepTile);
         hipFree(dDepVal);
         hipFree(dSig);
-        dTasks = nullptr;
-        dDepTile = dDepVal = dSig = nullptr;
     }
 };
+
+// Task DAGs depend only on (device, nt, b, alg, kend, tree, domain, update
+// tiles, workers); handles of the same shape share one device copy.
+struct ProgKey {
+    int device, nt, b, alg, kend, tree, domain, ut, workers;
+    bool operator<(const ProgKey &o) const
+    {
+        return std::tie(device, nt, b, alg, kend, tree, domain, ut, workers) <
+               std::tie(o.device, o.nt, o.b, o.alg, o.kend, o.tree, o.domain, o.ut, o.workers);
+    }
+};
+std::mutex g_prog_mu;
+std::map<ProgKey, std::shared_ptr<DevProgram>> g_progs;
 
 } // namespace
 
@@ -95,10 +109,9 @@ struct avs_handle {
     bool have_matrix = false;
     int last_alg = -1, last_kend = 0;
     bool timed = false;
-    int workers = 0;
+    int workers = 0, device = 0;
     long long wallclock_khz = 100000;
-    DevProgram prog;
-    bool prog_valid = false;
+    std::shared_ptr<DevProgram> prog;
 };
 
 namespace {
@@ -115,40 +128,45 @@ int upd_tiles_for(int b) { return b >= 128 ? 1 : (b == 64 ? 2 : 4); }
 avs_status ensure_program(avs_handle *h, int alg, int kend)
 {
     const int tree = resolve_tree(h), dom = resolve_domain(h), ut = upd_tiles_for(h->b);
-    if (h->prog_valid && h->prog.P.alg == alg && h->prog.P.kend == kend && h->prog.P.tree == tree &&
-        h->prog.P.domain == dom)
+    const ProgKey key{h->device, h->nt, h->b, alg, kend, tree, dom, ut, h->workers};
+    if (h->prog && h->prog->P.alg == alg && h->prog->P.kend == kend && h->prog->P.tree == tree &&
+        h->prog->P.domain == dom)
         return AVS_OK;
-    h->prog.release();
-    h->prog_valid = false;
-    build_program(h->prog.P, h->nt, h->b, alg, kend, tree, dom, ut, h->workers);
-    Program &P = h->prog.P;
-    if (h->opts.sched == AVS_SCHED_PERSISTENT) {
-        std::vector<Task> ordered(P.order.size());
-        for (size_t i = 0; i < P.order.size(); ++i) ordered[i] = P.prog[P.order[i]];
-        const size_t nd = std::max<size_t>(P.dep_tile.size(), 1), ns = std::max<size_t>(P.sig_tile.size(), 1);
-        HIPCHK(hipMalloc(&h->prog.dTasks, sizeof(Task) * std::max<size_t>(ordered.size(), 1)));
-        HIPCHK(hipMalloc(&h->prog.dDepTile, sizeof(int) * nd));
-        HIPCHK(hipMalloc(&h->prog.dDepVal, sizeof(int) * nd));
-        HIPCHK(hipMalloc(&h->prog.dSig, sizeof(int) * ns));
-        if (!ordered.empty())
-            HIPCHK(hipMemcpy(h->prog.dTasks, ordered.data(), sizeof(Task) * ordered.size(), hipMemcpyHostToDevice));
-        if (!P.dep_tile.empty()) {
-            HIPCHK(hipMemcpy(h->prog.dDepTile, P.dep_tile.data(), sizeof(int) * P.dep_tile.size(),
-                             hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(h->prog.dDepVal, P.dep_val.data(), sizeof(int) * P.dep_val.size(),
-                             hipMemcpyHostToDevice));
-        }
-        if (!P.sig_tile.empty())
-            HIPCHK(hipMemcpy(h->prog.dSig, P.sig_tile.data(), sizeof(int) * P.sig_tile.size(),
-                             hipMemcpyHostToDevice));
+    std::lock_guard<std::mutex> lk(g_prog_mu);
+    auto it = g_progs.find(key);
+    if (it != g_progs.end()) {
+        h->prog = it->second;
+        return AVS_OK;
     }
-    h->prog_valid = true;
+    auto dp = std::make_shared<DevProgram>();
+    dp->device = h->device;
+    build_program(dp->P, h->nt, h->b, alg, kend, tree, dom, ut, h->workers);
+    Program &P = dp->P;
+    std::vector<Task> ordered(P.order.size());
+    for (size_t i = 0; i < P.order.size(); ++i) ordered[i] = P.prog[P.order[i]];
+    const size_t nd = std::max<size_t>(P.dep_tile.size(), 1), ns = std::max<size_t>(P.sig_tile.size(), 1);
+    HIPCHK(hipMalloc(&dp->dTasks, sizeof(Task) * std::max<size_t>(ordered.size(), 1)));
+    HIPCHK(hipMalloc(&dp->dDepTile, sizeof(int) * nd));
+    HIPCHK(hipMalloc(&dp->dDepVal, sizeof(int) * nd));
+    HIPCHK(hipMalloc(&dp->dSig, sizeof(int) * ns));
+    if (!ordered.empty())
+        HIPCHK(hipMemcpy(dp->dTasks, ordered.data(), sizeof(Task) * ordered.size(), hipMemcpyHostToDevice));
+    if (!P.dep_tile.empty()) {
+        HIPCHK(hipMemcpy(dp->dDepTile, P.dep_tile.data(), sizeof(int) * P.dep_tile.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dp->dDepVal, P.dep_val.data(), sizeof(int) * P.dep_val.size(), hipMemcpyHostToDevice));
+    }
+    if (!P.sig_tile.empty())
+        HIPCHK(hipMemcpy(dp->dSig, P.sig_tile.data(), sizeof(int) * P.sig_tile.size(), hipMemcpyHostToDevice));
+    g_progs[key] = dp;
+    h->prog = dp;
     return AVS_OK;
 }
 
+int split_of(const avs_handle *h) { return h->dtype == AVS_F32 && h->opts.update_mode == AVS_UPD_SPLIT_BF16; }
+
 template <typename TS, int B> avs_status run_host_enqueued(avs_handle *h)
 {
-    const Program &P = h->prog.P;
+    const Program &P = h->prog->P;
     const long tsz = (long)B * B;
     TS *tiles = (TS *)h->dTiles;
     size_t i = 0;
@@ -171,7 +189,7 @@ template <typename TS, int B> avs_status run_host_enqueued(avs_handle *h)
         }
         const long st = (long)t.cstep * tsz;
         launch_update<TS, B>(t.mode, true, tiles + t.vtile * tsz, B, t.trans, tjj, tiles + t.c1tile * tsz, B, st,
-                             tiles + t.btile * tsz, B, st, t.trans, total, t.ntiles, h->stream);
+                             tiles + t.btile * tsz, B, st, t.trans, total, t.ntiles, split_of(h), h->stream);
         i = j;
     }
     HIPCHK(hipGetLastError());
@@ -180,15 +198,15 @@ template <typename TS, int B> avs_status run_host_enqueued(avs_handle *h)
 
 template <typename TS, int B> avs_status run_persistent(avs_handle *h)
 {
-    const Program &P = h->prog.P;
+    const Program &P = h->prog->P;
     HIPCHK(hipMemsetAsync(h->dCnt, 0, sizeof(int) * (size_t)h->nt * h->nt, h->stream));
     HIPCHK(hipMemsetAsync(h->dTicket, 0, sizeof(int), h->stream));
     HIPCHK(hipMemsetAsync(h->dStatus, 0, sizeof(int), h->stream));
     // dependency wait bound: opts.reserved[0] milliseconds (default 30 s)
     const long long ms = h->opts.reserved[0] > 0 ? h->opts.reserved[0] : 30000;
-    launch_persistent<TS, B>(h->workers, h->dTiles, h->dTau, h->dTjj, h->prog.dTasks, (int)P.order.size(),
-                             h->prog.dDepTile, h->prog.dDepVal, h->prog.dSig, h->dCnt, h->dTicket,
-                             ms * h->wallclock_khz, h->dStatus, h->stream);
+    launch_persistent<TS, B>(h->workers, h->dTiles, h->dTau, h->dTjj, h->prog->dTasks, (int)P.order.size(),
+                             h->prog->dDepTile, h->prog->dDepVal, h->prog->dSig, h->dCnt, h->dTicket,
+                             ms * h->wallclock_khz, h->dStatus, split_of(h), h->stream);
     HIPCHK(hipGetLastError());
     return AVS_OK;
 }
@@ -274,6 +292,7 @@ avs_status avs_create(avs_handle **hp, int32_t dtype, int64_t n, int32_t b, cons
     HIPCHK(hipEventCreate(&h->ev1));
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
+    h->device = dev;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     int wclk = 0;
     if (hipDeviceGetAttribute(&wclk, hipDeviceAttributeWallClockRate, dev) == hipSuccess && wclk > 0)
@@ -297,7 +316,7 @@ avs_status avs_destroy(avs_handle *h)
 {
     if (!h) return AVS_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
-    h->prog.release();
+    h->prog.reset();
     hipFree(h->dTiles);
     hipFree(h->dStage);
     hipFree(h->dTau);
@@ -341,6 +360,20 @@ avs_status avs_set_matrix_host(avs_handle *h, const void *A, int64_t lda)
     if (st) return st;
     const size_t es = esize(h->dtype);
     HIPCHK(hipMemcpy2DAsync(h->dStage, es * h->n, A, es * lda, es * h->n, h->n, hipMemcpyHostToDevice, h->stream));
+    return avs_set_matrix_device(h, h->dStage, h->n);
+}
+
+avs_status avs_set_matrix_host_sub(avs_handle *h, const void *A, int64_t m, int64_t lda)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!A) return fail(-2, "null matrix");
+    if (m <= 0 || m > h->n) return fail(-3, "m must be in 1..n");
+    if (lda < m) return fail(-4, "lda < m");
+    avs_status st = ensure_stage(h);
+    if (st) return st;
+    const size_t es = esize(h->dtype);
+    if (m < h->n) HIPCHK(hipMemsetAsync(h->dStage, 0, es * (size_t)h->n * h->n, h->stream));
+    HIPCHK(hipMemcpy2DAsync(h->dStage, es * h->n, A, es * lda, es * m, m, hipMemcpyHostToDevice, h->stream));
     return avs_set_matrix_device(h, h->dStage, h->n);
 }
 
@@ -478,7 +511,7 @@ static avs_status apply_op(int mode, int32_t dtype, int32_t b, const void *dV, i
         const long s1 = ctrans ? (long)B : (long)B * ldc1;
         const long s2 = ctrans ? (long)B : (long)B * ldc2;
         launch_update<TS, B>(mode, applyt != 0, dV, ldv, vtrans, dT, dC1, ldc1, s1, dC2, ldc2, s2, ctrans, ntl,
-                             upd_tiles_for(B), s);
+                             upd_tiles_for(B), 0, s);
         hipError_t e = hipGetLastError();
         hipStreamSynchronize(s);
         hipFree(dT);
@@ -592,7 +625,7 @@ struct avs_batch {
     double *dBands = nullptr, *dScratch = nullptr;
     long *dOff = nullptr, *dSoff = nullptr, *dTile0 = nullptr, *dCol0 = nullptr, *dBoff = nullptr;
     int *dNts = nullptr, *dOrder = nullptr, *dNext = nullptr;
-    int workers = 0;
+    int workers = 0, split = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool reduced = false, timed = false;
     std::vector<double> hBands;
@@ -674,6 +707,7 @@ avs_status avs_batch_create(avs_batch **bp, int32_t dtype, int32_t count, const 
     if (st) return st;
     if (occ < 1) return fail(AVS_ERR_HIP, "batch kernel does not fit on a CU");
     B->workers = std::min(ncu * occ, count);
+    B->split = (dtype == AVS_F32 && opts && opts->update_mode == AVS_UPD_SPLIT_BF16) ? 1 : 0;
     if (opts && opts->workers > 0) B->workers = std::min(B->workers, (int)opts->workers);
     HIPCHK(hipMalloc(&B->dScratch, sizeof(double) * 17 * b * (size_t)B->workers));
     HIPCHK(hipEventCreate(&B->ev0));
@@ -722,7 +756,7 @@ avs_status avs_batch_blockbidiag(avs_batch *B)
         HIPCHK(hipMemsetAsync(B->dNext, 0, sizeof(int), B->stream));
         HIPCHK(hipEventRecord(B->ev0, B->stream));
         launch_batch<TS, BB>(B->workers, B->dArena, B->dOff, B->dNts, B->dOrder, B->count, B->dScratch, B->dNext,
-                             B->stream);
+                             B->split, B->stream);
         HIPCHK(hipEventRecord(B->ev1, B->stream));
         HIPCHK(hipGetLastError());
         return AVS_OK;

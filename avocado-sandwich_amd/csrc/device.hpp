@@ -30,6 +30,10 @@ namespace avs {
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef unsigned short u8_t __attribute__((ext_vector_type(8)));
+
 template <typename T> struct AccT;
 template <> struct AccT<double> { typedef d4_t type; };
 template <> struct AccT<float> { typedef f4_t type; };
@@ -58,7 +62,11 @@ template <typename TS, int B> struct Lds {
     static constexpr int panel = B * (B + 1) / 2 + 2 * G + G * B + B + 4 + 16 * B + B + 16 * B;
     static constexpr int update = (B * (B + 1) * (int)sizeof(TS) + 16 * B * (int)sizeof(TS) + 7) / 8;
     static constexpr int tjj = 16 * B + B + 16 * B;
-    static constexpr int maxv = panel > update ? panel : update; // bodies run by the persistent executor
+    // split-BF16 update (FP32 storage): Y and Y^T as bf16 hi/lo pairs + T_JJ in FP32
+    static constexpr int SPLIT_S = B + 8;
+    static constexpr int update_split = sizeof(TS) == 4 ? (4 * B * SPLIT_S * 2 + 16 * B * 4 + 7) / 8 : 0;
+    static constexpr int upd = update > update_split ? update : update_split;
+    static constexpr int maxv = panel > upd ? panel : upd; // bodies run by the persistent executor
 };
 
 // ---------------------------------------------------------------------------
@@ -405,6 +413,176 @@ __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const 
             for (int m = 0; m < 4; ++m) c2[(long)(16 * tb + m) * rs2] = acc2[tb][m];
         }
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// Split-BF16 update for FP32 storage.  Every FP32 operand x of the two large
+// products is split as x = hi + lo with hi = bf16(x) and lo = bf16(x - hi)
+// (round to nearest even), and a*b is taken as hi*hi + hi*lo + lo*hi on the
+// bf16 matrix cores with FP32 accumulation (the lo*lo term, ~2^-16 relative,
+// is dropped).  Same blocking and K ordering as update_body:
+//   W  = C1[J] + Y_J^T C2   v_mfma_f32_16x16x32_bf16, K chunk p covers C2 rows
+//                           32p + 16(e/4) + 4q + e%4 for lane group q, e = 0..7
+//   W' = T_JJ^T W           native FP32 (v_mfma_f32_16x16x4f32)
+//   C2 -= Y_J W'            v_mfma_f32_16x16x16bf16_1k, K = 16 reflectors
+// Y is staged as bf16 hi/lo pairs twice: row-major for the C2 product and
+// transposed for the W product, so every A operand is two 8-byte LDS reads.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ void split_bf(float x, unsigned short &h, unsigned short &l)
+{
+    h = f2bf(x);
+    l = f2bf(x - bf2f(h));
+}
+
+template <int B, int MODE, bool APPLYT>
+__device__ void update_body_split(const float *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tjj,
+                                  float *__restrict__ C1, long ld1, long s1, float *__restrict__ C2, long ld2, long s2,
+                                  bool ctr, int ntiles, double *smem)
+{
+    typedef f4_t acc_t;
+    constexpr int S = B + 8;
+    constexpr int NB = B / 16;
+    unsigned short *YTh = reinterpret_cast<unsigned short *>(smem); // YT[i*S + k] = Y[k][i]
+    unsigned short *YTl = YTh + B * S;
+    unsigned short *Yh = YTl + B * S; // Y[k*S + i] = Y[k][i]
+    unsigned short *Yl = Yh + B * S;
+    float *TL = reinterpret_cast<float *>(Yl + B * S);
+    const int t = threadIdx.x;
+    for (int idx = t; idx < B * B; idx += NTHREADS) {
+        int k, i;
+        if (vtr) { i = idx % B; k = idx / B; } else { k = idx % B; i = idx / B; }
+        float v = V[toff(k, i, ldv, vtr)];
+        if (MODE == MODE_GE) v = (k < i) ? 0.f : (k == i ? 1.f : v);
+        if (MODE == MODE_TT && k > i) v = 0.f;
+        unsigned short h, l;
+        split_bf(v, h, l);
+        YTh[i * S + k] = h;
+        YTl[i * S + k] = l;
+        Yh[k * S + i] = h;
+        Yl[k * S + i] = l;
+    }
+    for (int idx = t; idx < 16 * B; idx += NTHREADS) TL[idx] = (float)tjj[idx];
+    __syncthreads();
+
+    const int lane = t & 63, w = t >> 6, q = lane >> 4, il = lane & 15;
+    const long rs1 = ctr ? ld1 : 1, cs1 = ctr ? 1 : ld1;
+    const long rs2 = ctr ? ld2 : 1, cs2 = ctr ? 1 : ld2;
+    const int nstrips = ntiles * NB;
+    for (int sidx = w; sidx < nstrips; sidx += NWAVES) {
+        const int tile = sidx / NB;
+        const int col = (sidx % NB) * 16 + il;
+        float *c1 = C1 + (long)tile * s1 + (long)col * cs1 + (long)(4 * q) * rs1;
+        float *c2 = C2 + (long)tile * s2 + (long)col * cs2 + (long)(4 * q) * rs2;
+        acc_t acc2[NB];
+#pragma unroll
+        for (int tb = 0; tb < NB; ++tb) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) acc2[tb][m] = c2[(long)(16 * tb + m) * rs2];
+        }
+#pragma unroll 1
+        for (int jj = 0; jj < NB; ++jj) {
+            const int J = APPLYT ? jj : (NB - 1 - jj);
+            acc_t c1v, Wa;
+            if (MODE != MODE_GE) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) c1v[m] = c1[(long)(16 * J + m) * rs1];
+                Wa = c1v;
+            } else {
+                Wa = vzero4<acc_t>();
+            }
+#pragma unroll
+            for (int p = 0; p < NB / 2; ++p) {
+                if (MODE == MODE_GE && 2 * p + 1 < J) continue;
+                if (MODE == MODE_TT && 2 * p > J) continue;
+                const unsigned short *th = YTh + (16 * J + il) * S + 32 * p + 4 * q;
+                const unsigned short *tl = YTl + (16 * J + il) * S + 32 * p + 4 * q;
+                u8_t ah, al, bh, bl;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    ah[e] = th[e];
+                    ah[e + 4] = th[16 + e];
+                    al[e] = tl[e];
+                    al[e + 4] = tl[16 + e];
+                    unsigned short h, l;
+                    split_bf(acc2[2 * p][e], h, l);
+                    bh[e] = h;
+                    bl[e] = l;
+                    split_bf(acc2[2 * p + 1][e], h, l);
+                    bh[e + 4] = h;
+                    bl[e + 4] = l;
+                }
+                const bf8_t Ah = __builtin_bit_cast(bf8_t, ah), Al = __builtin_bit_cast(bf8_t, al);
+                const bf8_t Bh = __builtin_bit_cast(bf8_t, bh), Bl = __builtin_bit_cast(bf8_t, bl);
+                Wa = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, Wa, 0, 0, 0);
+                Wa = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl, Wa, 0, 0, 0);
+                Wa = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, Wa, 0, 0, 0);
+            }
+            acc_t Wp = vzero4<acc_t>();
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const float av = APPLYT ? TL[(16 * J + 4 * q + m) * 16 + il] : TL[(16 * J + il) * 16 + 4 * q + m];
+                Wp = mfma16(av, Wa[m], Wp);
+            }
+            if (MODE != MODE_GE) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) c1[(long)(16 * J + m) * rs1] = c1v[m] - Wp[m];
+            }
+            s4_t wh, wl;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                unsigned short h, l;
+                split_bf(-Wp[e], h, l);
+                wh[e] = (short)h;
+                wl[e] = (short)l;
+            }
+#pragma unroll
+            for (int tb = 0; tb < NB; ++tb) {
+                if (MODE == MODE_GE && tb < J) continue;
+                if (MODE == MODE_TT && tb > J) continue;
+                const unsigned short *yh = Yh + (16 * tb + il) * S + 16 * J + 4 * q;
+                const unsigned short *yl = Yl + (16 * tb + il) * S + 16 * J + 4 * q;
+                s4_t ah, al;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    ah[e] = (short)yh[e];
+                    al[e] = (short)yl[e];
+                }
+                acc2[tb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, wh, acc2[tb], 0, 0, 0);
+                acc2[tb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, wl, acc2[tb], 0, 0, 0);
+                acc2[tb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, wh, acc2[tb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int tb = 0; tb < NB; ++tb) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) c2[(long)(16 * tb + m) * rs2] = acc2[tb][m];
+        }
+    }
+}
+
+// update with the arithmetic selected at run time: split-BF16 (FP32 storage
+// only) or the native MFMA of the storage type
+template <typename TS, int B, int MODE>
+__device__ __forceinline__ void update_sel(bool split, const TS *V, long ldv, bool vtr, const double *tjj, TS *C1,
+                                           long ld1, long s1, TS *C2, long ld2, long s2, bool ctr, int ntiles,
+                                           double *smem)
+{
+    if constexpr (sizeof(TS) == 4) {
+        if (split) {
+            update_body_split<B, MODE, true>((const float *)V, ldv, vtr, tjj, (float *)C1, ld1, s1, (float *)C2, ld2,
+                                             s2, ctr, ntiles, smem);
+            return;
+        }
+    }
+    update_body<TS, B, MODE, true>(V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, smem);
 }
 
 } // namespace avs

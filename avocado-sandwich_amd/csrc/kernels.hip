@@ -83,15 +83,20 @@ __global__ void __launch_bounds__(NTHREADS) tjj_kernel(const TS *V, long ldv, in
 template <typename TS, int B, int MODE, bool APPLYT>
 __global__ void __launch_bounds__(NTHREADS) update_kernel(const TS *V, long ldv, int vtr, const double *tjj,
                                                           TS *C1, long ld1, long s1, TS *C2, long ld2, long s2,
-                                                          int ctr, int ntiles_total, int per_block)
+                                                          int ctr, int ntiles_total, int per_block, int split)
 {
-    __shared__ double smem[Lds<TS, B>::update];
+    __shared__ double smem[Lds<TS, B>::upd];
     const int first = blockIdx.x * per_block;
     int cnt = ntiles_total - first;
     if (cnt > per_block) cnt = per_block;
     if (cnt <= 0) return;
-    update_body<TS, B, MODE, APPLYT>(V, ldv, vtr != 0, tjj, C1 + (long)first * s1, ld1, s1, C2 + (long)first * s2,
-                                     ld2, s2, ctr != 0, cnt, smem);
+    if constexpr (APPLYT) {
+        update_sel<TS, B, MODE>(split != 0, V, ldv, vtr != 0, tjj, C1 + (long)first * s1, ld1, s1,
+                                C2 + (long)first * s2, ld2, s2, ctr != 0, cnt, smem);
+    } else {
+        update_body<TS, B, MODE, APPLYT>(V, ldv, vtr != 0, tjj, C1 + (long)first * s1, ld1, s1, C2 + (long)first * s2,
+                                         ld2, s2, ctr != 0, cnt, smem);
+    }
 }
 
 // ------------------------------------------------------ persistent executor
@@ -108,7 +113,8 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
                                                                   const int *__restrict__ dep_tile,
                                                                   const int *__restrict__ dep_val,
                                                                   const int *__restrict__ sig_tile, int *cnt,
-                                                                  int *ticket, long long wait_ticks, int *status)
+                                                                  int *ticket, long long wait_ticks, int *status,
+                                                                  int split)
 {
     __shared__ double smem[Lds<TS, B>::maxv];
     __shared__ int sh_task;
@@ -164,11 +170,11 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
             TS *C1 = tiles + (long)T.c1tile * tsz;
             const long st = (long)T.cstep * tsz;
             if (T.mode == MODE_GE)
-                update_body<TS, B, MODE_GE, true>(V, B, tr, tjj, C2, B, st, C2, B, st, tr, T.ntiles, smem);
+                update_sel<TS, B, MODE_GE>(split, V, B, tr, tjj, C2, B, st, C2, B, st, tr, T.ntiles, smem);
             else if (T.mode == MODE_TS)
-                update_body<TS, B, MODE_TS, true>(V, B, tr, tjj, C1, B, st, C2, B, st, tr, T.ntiles, smem);
+                update_sel<TS, B, MODE_TS>(split, V, B, tr, tjj, C1, B, st, C2, B, st, tr, T.ntiles, smem);
             else
-                update_body<TS, B, MODE_TT, true>(V, B, tr, tjj, C1, B, st, C2, B, st, tr, T.ntiles, smem);
+                update_sel<TS, B, MODE_TT>(split, V, B, tr, tjj, C1, B, st, C2, B, st, tr, T.ntiles, smem);
         }
         __builtin_amdgcn_s_waitcnt(0); // this thread's global stores are acknowledged
         __syncthreads();
@@ -194,7 +200,7 @@ template <typename TS, int B>
 __global__ void __launch_bounds__(NTHREADS, 1) batch_kernel(TS *arena, const long *__restrict__ off,
                                                              const int *__restrict__ nts,
                                                              const int *__restrict__ order, int count,
-                                                             double *scratch, int *next)
+                                                             double *scratch, int *next, int split)
 {
     __shared__ double smem[Lds<TS, B>::maxv];
     __shared__ int sh_m;
@@ -217,29 +223,29 @@ __global__ void __launch_bounds__(NTHREADS, 1) batch_kernel(TS *arena, const lon
             panel_body<TS, B, MODE_GE>(tile(k, k), B, tile(k, k), B, false, tau, tjj, smem);
             __syncthreads();
             if (nc > 0)
-                update_body<TS, B, MODE_GE, true>(tile(k, k), B, false, tjj, tile(k, k + 1), B, nt * tsz, tile(k, k + 1),
-                                                   B, nt * tsz, false, nc, smem);
+                update_sel<TS, B, MODE_GE>(split, tile(k, k), B, false, tjj, tile(k, k + 1), B, nt * tsz,
+                                           tile(k, k + 1), B, nt * tsz, false, nc, smem);
             __syncthreads();
             for (int i = k + 1; i < nt; ++i) {
                 panel_body<TS, B, MODE_TS>(tile(k, k), B, tile(i, k), B, false, tau, tjj, smem);
                 __syncthreads();
                 if (nc > 0)
-                    update_body<TS, B, MODE_TS, true>(tile(i, k), B, false, tjj, tile(k, k + 1), B, nt * tsz,
-                                                       tile(i, k + 1), B, nt * tsz, false, nc, smem);
+                    update_sel<TS, B, MODE_TS>(split, tile(i, k), B, false, tjj, tile(k, k + 1), B, nt * tsz,
+                                               tile(i, k + 1), B, nt * tsz, false, nc, smem);
                 __syncthreads();
             }
             if (k == nt - 1) break;
             // LQ sweep on tile row k from column k+1 (adjoint addressing)
             panel_body<TS, B, MODE_GE>(tile(k, k + 1), B, tile(k, k + 1), B, true, tau, tjj, smem);
             __syncthreads();
-            update_body<TS, B, MODE_GE, true>(tile(k, k + 1), B, true, tjj, tile(k + 1, k + 1), B, tsz,
-                                               tile(k + 1, k + 1), B, tsz, true, nc, smem);
+            update_sel<TS, B, MODE_GE>(split, tile(k, k + 1), B, true, tjj, tile(k + 1, k + 1), B, tsz,
+                                       tile(k + 1, k + 1), B, tsz, true, nc, smem);
             __syncthreads();
             for (int j = k + 2; j < nt; ++j) {
                 panel_body<TS, B, MODE_TS>(tile(k, k + 1), B, tile(k, j), B, true, tau, tjj, smem);
                 __syncthreads();
-                update_body<TS, B, MODE_TS, true>(tile(k, j), B, true, tjj, tile(k + 1, k + 1), B, tsz, tile(k + 1, j),
-                                                   B, tsz, true, nc, smem);
+                update_sel<TS, B, MODE_TS>(split, tile(k, j), B, true, tjj, tile(k + 1, k + 1), B, tsz,
+                                           tile(k + 1, j), B, tsz, true, nc, smem);
                 __syncthreads();
             }
         }
@@ -302,10 +308,10 @@ __global__ void __launch_bounds__(256) batch_band_kernel(const TS *__restrict__ 
 
 template <typename TS, int B>
 void launch_batch(int workers, void *arena, const long *off, const int *nts, const int *order, int count,
-                  double *scratch, int *next, hipStream_t s)
+                  double *scratch, int *next, int split, hipStream_t s)
 {
     hipLaunchKernelGGL((batch_kernel<TS, B>), dim3(workers), dim3(NTHREADS), 0, s, (TS *)arena, off, nts, order, count,
-                       scratch, next);
+                       scratch, next, split);
 }
 template <typename TS, int B> int batch_occupancy()
 {
@@ -391,37 +397,41 @@ void launch_tjj(int mode, const void *V, long ldv, int vtr, const double *tau, d
 
 template <typename TS, int B, int MODE>
 static void launch_update_m(bool applyt, const void *V, long ldv, int vtr, const double *tjj, void *C1, long ld1,
-                            long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, hipStream_t s)
+                            long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, int split,
+                            hipStream_t s)
 {
     const int nblk = (ntiles + per_block - 1) / per_block;
     if (nblk <= 0) return;
     if (applyt)
         hipLaunchKernelGGL((update_kernel<TS, B, MODE, true>), dim3(nblk), dim3(NTHREADS), 0, s, (const TS *)V, ldv,
-                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block);
+                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block, split);
     else
         hipLaunchKernelGGL((update_kernel<TS, B, MODE, false>), dim3(nblk), dim3(NTHREADS), 0, s, (const TS *)V, ldv,
-                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block);
+                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block, 0);
 }
 
 template <typename TS, int B>
 void launch_update(int mode, bool applyt, const void *V, long ldv, int vtr, const double *tjj, void *C1, long ld1,
-                   long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, hipStream_t s)
+                   long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, int split, hipStream_t s)
 {
     if (mode == MODE_GE)
-        launch_update_m<TS, B, MODE_GE>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block, s);
+        launch_update_m<TS, B, MODE_GE>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block,
+                                        split, s);
     else if (mode == MODE_TS)
-        launch_update_m<TS, B, MODE_TS>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block, s);
+        launch_update_m<TS, B, MODE_TS>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block,
+                                        split, s);
     else
-        launch_update_m<TS, B, MODE_TT>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block, s);
+        launch_update_m<TS, B, MODE_TT>(applyt, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, per_block,
+                                        split, s);
 }
 
 template <typename TS, int B>
 void launch_persistent(int workers, void *tiles, double *taus, double *tjjs, const Task *tasks, int ntasks,
                        const int *dep_tile, const int *dep_val, const int *sig_tile, int *cnt, int *ticket,
-                       long long wait_ticks, int *status, hipStream_t s)
+                       long long wait_ticks, int *status, int split, hipStream_t s)
 {
     hipLaunchKernelGGL((persistent_kernel<TS, B>), dim3(workers), dim3(NTHREADS), 0, s, (TS *)tiles, taus, tjjs, tasks,
-                       ntasks, dep_tile, dep_val, sig_tile, cnt, ticket, wait_ticks, status);
+                       ntasks, dep_tile, dep_val, sig_tile, cnt, ticket, wait_ticks, status, split);
 }
 
 template <typename TS, int B> int persistent_occupancy()
@@ -442,11 +452,12 @@ void launch_mfma_probe(const double *a, const double *b, double *o64, const floa
     template void launch_panel<TS, B>(int, void *, long, void *, long, int, double *, double *, hipStream_t);         \
     template void launch_tjj<TS, B>(int, const void *, long, int, const double *, double *, hipStream_t);             \
     template void launch_update<TS, B>(int, bool, const void *, long, int, const double *, void *, long, long,        \
-                                       void *, long, long, int, int, int, hipStream_t);                                \
+                                       void *, long, long, int, int, int, int, hipStream_t);                           \
     template void launch_persistent<TS, B>(int, void *, double *, double *, const Task *, int, const int *,           \
-                                           const int *, const int *, int *, int *, long long, int *, hipStream_t);    \
+                                           const int *, const int *, int *, int *, long long, int *, int,             \
+                                           hipStream_t);                                                               \
     template int persistent_occupancy<TS, B>();                                                                        \
-    template void launch_batch<TS, B>(int, void *, const long *, const int *, const int *, int, double *, int *,      \
+    template void launch_batch<TS, B>(int, void *, const long *, const int *, const int *, int, double *, int *, int, \
                                       hipStream_t);                                                                    \
     template int batch_occupancy<TS, B>();
 

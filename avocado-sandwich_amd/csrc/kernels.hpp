@@ -19,16 +19,17 @@ template <typename TS, int B>
 void launch_tjj(int mode, const void *V, long ldv, int vtr, const double *tau, double *tjj, hipStream_t s);
 template <typename TS, int B>
 void launch_update(int mode, bool applyt, const void *V, long ldv, int vtr, const double *tjj, void *C1, long ld1,
-                   long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, hipStream_t s);
+                   long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, int split,
+                   hipStream_t s);
 template <typename TS, int B>
 void launch_persistent(int workers, void *tiles, double *taus, double *tjjs, const Task *tasks, int ntasks,
                        const int *dep_tile, const int *dep_val, const int *sig_tile, int *cnt, int *ticket,
-                       long long wait_ticks, int *status, hipStream_t s);
+                       long long wait_ticks, int *status, int split, hipStream_t s);
 template <typename TS, int B> int persistent_occupancy();
 
 template <typename TS, int B>
 void launch_batch(int workers, void *arena, const long *off, const int *nts, const int *order, int count,
-                  double *scratch, int *next, hipStream_t s);
+                  double *scratch, int *next, int split, hipStream_t s);
 template <typename TS, int B> int batch_occupancy();
 template <typename TS>
 void launch_batch_pack(const void *stage, const long *soff, void *arena, const long *off, const int *nts,

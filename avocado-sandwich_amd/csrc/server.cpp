@@ -1,0 +1,299 @@
+// server.cpp -- continuous-batching svdvals request loop.
+//
+// Requests (any n, FP32 or FP64) are accepted as they arrive.  A request is
+// admitted to the device when the HBM budget has room for its working set
+// (tiles, staging copy, factor slots); admission is first-come first-served.
+// Each of `max_inflight` device workers owns its own HIP stream, so the band
+// reductions of admitted requests overlap on the GPU while later requests
+// wait for memory.  Only the (b+1) x N band comes back over PCIe; the host
+// stage (block bulge chasing + dbdsdc, FP64) runs on a pool of host workers.
+// Latency is arrival -> singular values ready.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "avsvd.h"
+#include "internal.hpp"
+
+using avs::fail;
+
+namespace {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    int dtype = AVS_F64;
+    int64_t n = 0, N = 0;
+    int b = 32;
+    size_t footprint = 0;
+    std::vector<char> A;
+    std::vector<double> band, sigma;
+    double t_arrive = 0, t_admit = 0, t_band = 0, t_done = 0;
+    avs_status status = AVS_OK;
+    std::string err;
+    bool done = false;
+};
+
+int pick_b(const avs_server_opts &o, int dtype, int64_t n)
+{
+    const int want = dtype == AVS_F64 ? o.b_f64 : o.b_f32;
+    if (want == 32 || want == 64 || want == 128) return want;
+    // the host stage cost grows with b; small tiles keep it cheap
+    return n <= 2048 ? 32 : 64;
+}
+
+size_t footprint_of(int dtype, int64_t N, int b)
+{
+    const size_t es = dtype == AVS_F64 ? 8 : 4;
+    const size_t nt = (size_t)(N / b);
+    return 2 * es * (size_t)N * N + 4 * nt * nt * 17 * (size_t)b * 8 + nt * nt * 4 + (size_t)(b + 1) * N * 8 +
+           (64u << 20);
+}
+
+} // namespace
+
+struct avs_server {
+    avs_server_opts opts{};
+    std::mutex mu;
+    std::condition_variable cv_dev, cv_host, cv_done;
+    std::deque<std::shared_ptr<Request>> pending, host_q;
+    std::map<uint64_t, std::shared_ptr<Request>> reqs;
+    size_t used = 0;
+    bool stop = false;
+    uint64_t next_id = 1;
+    std::vector<std::thread> dev_workers, host_workers;
+    std::vector<double> lat;
+    double t_first = -1, t_last = 0;
+    int64_t completed = 0;
+};
+
+namespace {
+
+void device_worker(avs_server *S)
+{
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_dev.wait(lk, [&] {
+                return S->stop || (!S->pending.empty() &&
+                                   S->used + S->pending.front()->footprint <= (size_t)S->opts.hbm_budget_bytes);
+            });
+            if (S->stop) return;
+            r = S->pending.front();
+            S->pending.pop_front();
+            S->used += r->footprint;
+            r->t_admit = now_ms();
+        }
+        avs_opts o;
+        avs_default_opts(&o);
+        o.update_mode = S->opts.update_mode;
+        const long nt = (long)(r->N / r->b);
+        o.workers = (int32_t)std::max(16L, std::min(256L, nt * nt / 2));
+        avs_handle *h = nullptr;
+        avs_status st = avs_create(&h, r->dtype, r->N, r->b, &o);
+        if (!st) st = avs_set_matrix_host_sub(h, r->A.data(), r->n, r->n);
+        if (!st) st = avs_blockbidiag(h, 0);
+        if (!st) {
+            r->band.assign((size_t)(r->b + 1) * r->N, 0.0);
+            st = avs_get_band(h, r->band.data(), r->b + 1);
+        }
+        std::string msg = st ? avs_last_error() : "";
+        if (h) avs_destroy(h);
+        r->t_band = now_ms();
+        std::vector<char>().swap(r->A);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->used -= r->footprint;
+            r->status = st;
+            r->err = msg;
+            S->host_q.push_back(r);
+        }
+        S->cv_dev.notify_all();
+        S->cv_host.notify_one();
+    }
+}
+
+void host_worker(avs_server *S)
+{
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_host.wait(lk, [&] { return S->stop || !S->host_q.empty(); });
+            if (S->host_q.empty()) return;
+            r = S->host_q.front();
+            S->host_q.pop_front();
+        }
+        if (!r->status) {
+            const int64_t N = r->N;
+            std::vector<double> d(N), e(std::max<int64_t>(N - 1, 1)), sg(N);
+            const int threads = N >= 4096 ? 4 : 1;
+            avs_status st = avs::bidiagonalize_band_threads(N, r->b, r->band.data(), r->b + 1, d.data(), e.data(),
+                                                           threads);
+            if (!st) st = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
+            if (st) {
+                r->status = st;
+                r->err = avs_last_error();
+            } else {
+                r->sigma.assign(sg.begin(), sg.begin() + r->n); // drop the padding's zeros
+            }
+        }
+        std::vector<double>().swap(r->band);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            r->t_done = now_ms();
+            r->done = true;
+            if (!r->status) {
+                S->lat.push_back(r->t_done - r->t_arrive);
+                S->t_last = std::max(S->t_last, r->t_done);
+                S->completed++;
+            }
+        }
+        S->cv_done.notify_all();
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+void avs_server_default_opts(avs_server_opts *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->hbm_budget_bytes = (int64_t)64 << 30;
+    o->max_inflight = 4;
+    o->host_threads = avs::host_thread_count();
+    o->update_mode = AVS_UPD_NATIVE;
+}
+
+avs_status avs_server_create(avs_server **sp, const avs_server_opts *opts)
+{
+    if (!sp) return fail(-1, "null server pointer");
+    *sp = nullptr;
+    std::unique_ptr<avs_server> S(new avs_server());
+    if (opts) S->opts = *opts;
+    else avs_server_default_opts(&S->opts);
+    if (S->opts.hbm_budget_bytes <= 0) return fail(-2, "hbm_budget_bytes must be positive");
+    if (S->opts.max_inflight <= 0) S->opts.max_inflight = 4;
+    if (S->opts.host_threads <= 0) S->opts.host_threads = avs::host_thread_count();
+    for (int i = 0; i < S->opts.max_inflight; ++i) S->dev_workers.emplace_back(device_worker, S.get());
+    for (int i = 0; i < S->opts.host_threads; ++i) S->host_workers.emplace_back(host_worker, S.get());
+    *sp = S.release();
+    return AVS_OK;
+}
+
+avs_status avs_server_submit(avs_server *S, int32_t dtype, int64_t n, const void *A, int64_t lda, uint64_t *req_id)
+{
+    if (!S) return fail(-1, "null server");
+    if (dtype != AVS_F64 && dtype != AVS_F32) return fail(-2, "dtype %d not supported", dtype);
+    if (n <= 0) return fail(-3, "n must be positive");
+    if (!A) return fail(-4, "null matrix");
+    if (lda < n) return fail(-5, "lda < n");
+    if (!req_id) return fail(-6, "null request id");
+    auto r = std::make_shared<Request>();
+    r->t_arrive = now_ms();
+    r->dtype = dtype;
+    r->n = n;
+    r->b = pick_b(S->opts, dtype, n);
+    r->N = (n + r->b - 1) / r->b * r->b;
+    r->footprint = footprint_of(dtype, r->N, r->b);
+    if (r->footprint > (size_t)S->opts.hbm_budget_bytes)
+        return fail(AVS_ERR_BUDGET, "request n=%lld needs %zu bytes, budget %lld", (long long)n, r->footprint,
+                    (long long)S->opts.hbm_budget_bytes);
+    const size_t es = dtype == AVS_F64 ? 8 : 4;
+    r->A.resize(es * (size_t)n * n);
+    for (int64_t j = 0; j < n; ++j)
+        memcpy(r->A.data() + es * (size_t)j * n, (const char *)A + es * (size_t)j * lda, es * n);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->id = S->next_id++;
+        if (S->t_first < 0) S->t_first = r->t_arrive;
+        S->reqs[r->id] = r;
+        S->pending.push_back(r);
+    }
+    S->cv_dev.notify_all();
+    *req_id = r->id;
+    return AVS_OK;
+}
+
+avs_status avs_server_poll(avs_server *S, uint64_t req_id, int32_t *done)
+{
+    if (!S) return fail(-1, "null server");
+    if (!done) return fail(-3, "null output");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(req_id);
+    if (it == S->reqs.end()) return fail(-2, "unknown request %llu", (unsigned long long)req_id);
+    *done = it->second->done ? 1 : 0;
+    return AVS_OK;
+}
+
+avs_status avs_server_wait(avs_server *S, uint64_t req_id, double *sigma, double *latency_ms)
+{
+    if (!S) return fail(-1, "null server");
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->reqs.find(req_id);
+        if (it == S->reqs.end()) return fail(-2, "unknown request %llu", (unsigned long long)req_id);
+        r = it->second;
+        S->cv_done.wait(lk, [&] { return r->done; });
+        S->reqs.erase(it);
+    }
+    if (r->status) return fail(r->status, "request %llu failed: %s", (unsigned long long)req_id, r->err.c_str());
+    if (sigma) memcpy(sigma, r->sigma.data(), sizeof(double) * r->n);
+    if (latency_ms) *latency_ms = r->t_done - r->t_arrive;
+    return AVS_OK;
+}
+
+avs_status avs_server_stats(avs_server *S, double *p50_ms, double *p99_ms, double *solves_per_s, int64_t *completed)
+{
+    if (!S) return fail(-1, "null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::vector<double> l = S->lat;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) {
+        if (l.empty()) return 0.0;
+        const size_t i = std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5));
+        return l[i];
+    };
+    if (p50_ms) *p50_ms = pct(0.50);
+    if (p99_ms) *p99_ms = pct(0.99);
+    if (solves_per_s) *solves_per_s = (S->completed > 0 && S->t_last > S->t_first)
+                                          ? 1000.0 * S->completed / (S->t_last - S->t_first) : 0.0;
+    if (completed) *completed = S->completed;
+    return AVS_OK;
+}
+
+avs_status avs_server_destroy(avs_server *S)
+{
+    if (!S) return AVS_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_dev.notify_all();
+    S->cv_host.notify_all();
+    for (auto &t : S->dev_workers) t.join();
+    for (auto &t : S->host_workers) t.join();
+    delete S;
+    return AVS_OK;
+}
+
+} // extern "C"

@@ -91,6 +91,10 @@ avs_status avs_destroy(avs_handle *h);
 /* Load the matrix (element type = the handle's dtype), column-major, lda >= n. */
 avs_status avs_set_matrix_host(avs_handle *h, const void *A, int64_t lda);
 avs_status avs_set_matrix_device(avs_handle *h, const void *dA, int64_t lda);
+/* Load an m x m matrix (m <= n) into the leading block and zero the rest:
+ * the zero padding only adds n - m zero singular values.  For sizes that are
+ * not multiples of b (the reference rejects them, TiledMatrix.jl:42-44). */
+avs_status avs_set_matrix_host_sub(avs_handle *h, const void *A, int64_t m, int64_t lda);
 
 /* Read back the reduced n x n matrix in the reference's in-place layout:
  * after Blockbidiag!, the band (upper-triangular diagonal tiles, lower-
@@ -186,6 +190,34 @@ avs_status avs_batch_get_band(avs_batch *batch, int32_t idx, double *band, int64
 /* All singular values, concatenated in matrix order (n_i values each,
  * descending); runs the reduction first if needed. */
 avs_status avs_batch_svdvals(avs_batch *batch, double *sigma);
+
+/* ---- serving: continuous-batching svdvals request loop ----
+ * Requests of any n (FP32 or FP64) are accepted as they arrive and admitted
+ * to the device first-come first-served while the HBM budget has room for
+ * their working set; up to max_inflight band reductions run concurrently on
+ * separate streams.  Only the band crosses PCIe; the host stage runs on
+ * host_threads workers.  Latency = submit -> singular values ready.  No
+ * reference counterpart (the reference has no serving path). */
+typedef struct avs_server avs_server;
+typedef struct avs_server_opts {
+    int64_t hbm_budget_bytes; /* device working-set budget for admitted requests */
+    int32_t max_inflight;     /* concurrent device reductions (streams) */
+    int32_t host_threads;     /* host-stage workers */
+    int32_t b_f32, b_f64;     /* tile sizes (0 = 32 up to n=2048, else 64) */
+    int32_t update_mode;      /* avs_update_mode for FP32 requests */
+    int32_t reserved[7];
+} avs_server_opts;
+void avs_server_default_opts(avs_server_opts *opts);
+avs_status avs_server_create(avs_server **server, const avs_server_opts *opts);
+/* Copies A (n x n, column-major, dtype) and queues it; returns its id. */
+avs_status avs_server_submit(avs_server *server, int32_t dtype, int64_t n, const void *A, int64_t lda,
+                             uint64_t *req_id);
+avs_status avs_server_poll(avs_server *server, uint64_t req_id, int32_t *done);
+/* Blocks until the request is done; sigma gets n values (descending). */
+avs_status avs_server_wait(avs_server *server, uint64_t req_id, double *sigma, double *latency_ms);
+avs_status avs_server_stats(avs_server *server, double *p50_ms, double *p99_ms, double *solves_per_s,
+                            int64_t *completed);
+avs_status avs_server_destroy(avs_server *server);
 
 /* ---- diagnostics ----
  * avs_debug_mfma_layout: runs one v_mfma_f64_16x16x4f64 and one

@@ -50,6 +50,19 @@ def test_mfma_lane_layout(gpu):
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("b", [32, 64])
+def test_geqrt_vs_lapack(gpu, golden_dir, dt, b):
+    """panel kernel alone (no matrix cores)"""
+    import torch
+    F = np.load(os.path.join(golden_dir, "kernels_b%d.npz" % b))
+    A = dev(F["a"], dt, gpu)
+    tau = torch.zeros(b, dtype=torch.float64, device=gpu)
+    avsvd.geqrt_(DT[dt], b, ptr(A), b, ptr(tau))
+    np.testing.assert_allclose(host(A), F["geqrt_factors"], atol=tol(dt, b) * 10)
+    np.testing.assert_allclose(tau.cpu().numpy()[: b - 1], F["geqrt_tau"][: b - 1], atol=tol(dt, b))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("b", [32, 64])
 def test_geqrt_and_unmqr_vs_lapack(gpu, golden_dir, dt, b):
     import torch
     F = np.load(os.path.join(golden_dir, "kernels_b%d.npz" % b))

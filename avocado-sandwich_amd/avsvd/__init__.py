@@ -16,7 +16,7 @@ same C ABI with ccall (avocado-sandwich_amd/julia/AvSVD.jl):
 Julia's `!` suffix becomes a trailing underscore; every such function
 mutates its argument in place like the reference.
 """
-from ._lib import (AVS_F32, AVS_F64, SCHED_HOST, SCHED_PERSISTENT, TREE_FLAT, TREE_HIER, UPD_NATIVE,
+from ._lib import (AVS_F32, AVS_F64, SCHED_HOST, SCHED_PERSISTENT, SCHED_STREAMS, TREE_FLAT, TREE_HIER, UPD_NATIVE,
                    UPD_SPLIT_BF16, AvsError, EXPORTS, LIB_PATH, lib)
 from .batch import Batch, batch_svdvals
 from .server import Server
@@ -29,6 +29,6 @@ __all__ = [
     "TiledMatrix", "Blockbidiag_", "tiled_QR_", "OOC_QR_", "OOC_Bidiag_", "OOC_SVD_", "compile_kernels",
     "bidiagonalize", "bidiagonalize_band", "diag_lapack", "bdsdc_values", "my_CU_svdval", "svdvals",
     "gaussian_matrix", "geqrt_", "tsqrt_", "unmqr_", "tsmqr_", "schedule_stats", "schedule_dump", "mfma_layout",
-    "AvsError", "EXPORTS", "LIB_PATH", "lib", "AVS_F32", "AVS_F64", "SCHED_HOST", "SCHED_PERSISTENT",
+    "AvsError", "EXPORTS", "LIB_PATH", "lib", "AVS_F32", "AVS_F64", "SCHED_HOST", "SCHED_PERSISTENT", "SCHED_STREAMS",
     "TREE_FLAT", "TREE_HIER", "UPD_NATIVE", "UPD_SPLIT_BF16",
 ]

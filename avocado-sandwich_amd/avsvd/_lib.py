@@ -13,7 +13,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libavsvd.so")
 
 AVS_F64, AVS_F32 = 0, 1
-SCHED_PERSISTENT, SCHED_HOST = 0, 1
+SCHED_PERSISTENT, SCHED_HOST, SCHED_STREAMS = 0, 1, 2
 TREE_FLAT, TREE_HIER = 0, 1
 UPD_NATIVE, UPD_SPLIT_BF16 = 0, 1
 
@@ -84,7 +84,7 @@ def lib():
             "avs_unmqr": [i32, i32, vp, i64, vp, vp, i64, i64, i32, i32, i32, vp],
             "avs_tsmqr": [i32, i32, vp, i64, vp, i64, vp, i64, vp, i64, i32, i32, i32, vp],
             "avs_gen_gaussian_host": [ctypes.c_uint64, i64, ctypes.c_double, i32, vp, i64],
-            "avs_debug_mfma_layout": [dp, ctypes.POINTER(ctypes.c_float)],
+            "avs_debug_mfma_layout": [i32, dp, ctypes.POINTER(ctypes.c_float)],
             "avs_schedule_stats": [i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(i64), dp],
             "avs_schedule_dump": [i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int32), i64,
                                   ctypes.POINTER(i64)],

@@ -271,8 +271,8 @@ def schedule_dump(nt, b, alg=0, kend=0, tree=1, domain=16, workers=256):
     return out
 
 
-def mfma_layout():
+def mfma_layout(variant=0):
     o64 = np.zeros(256)
     o32 = np.zeros(256, dtype=np.float32)
-    check(lib().avs_debug_mfma_layout(dptr(o64), o32.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    check(lib().avs_debug_mfma_layout(variant, dptr(o64), o32.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
     return o64.reshape(64, 4), o32.reshape(64, 4)

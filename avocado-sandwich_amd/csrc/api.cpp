@@ -214,6 +214,10 @@ template <typename TS, int B> avs_status run_persistent(avs_handle *h)
 avs_status run_alg(avs_handle *h, int alg, int32_t kend)
 {
     if (!h) return fail(-1, "null handle");
+    if (h->opts.sched == AVS_SCHED_PERSISTENT)
+        return fail(AVS_ERR_UNSUPPORTED,
+                    "the persistent dependency-counting executor is disabled: it did not complete on the device "
+                    "and its cause is not yet identified; use AVS_SCHED_STREAMS or AVS_SCHED_HOST");
     if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
     if (kend <= 0 || kend > h->nt) kend = h->nt;
     avs_status st = ensure_program(h, alg, kend);
@@ -253,7 +257,7 @@ void avs_default_opts(avs_opts *o)
 {
     if (!o) return;
     memset(o, 0, sizeof *o);
-    o->sched = AVS_SCHED_PERSISTENT;
+    o->sched = AVS_SCHED_HOST;
     o->tree = AVS_TREE_HIER;
     o->domain = 0;
     o->update_mode = AVS_UPD_NATIVE;
@@ -545,14 +549,17 @@ avs_status avs_tsmqr(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC
     return apply_op(MODE_TS, dtype, b, dV, ldv, dtau, dC1, ldc1, dC2, ldc2, ncols, applyt, vtrans, ctrans, stream);
 }
 
-avs_status avs_debug_mfma_layout(double *out64, float *out32)
+avs_status avs_debug_mfma_layout(int32_t variant, double *out64, float *out32)
 {
-    if (!out64 || !out32) return fail(-1, "null output");
+    if (variant < 0 || variant > 2) return fail(-1, "variant must be 0, 1 or 2");
+    if (!out64 || !out32) return fail(-2, "null output");
     double ha[64], hb[64];
     float fa[64], fb[64];
     for (int i = 0; i < 64; ++i) {
-        ha[i] = (double)(i + 1);
-        hb[i] = (double)((i * 7) % 13 - 6);
+        // lane i supplies ha[i] as its A operand and hb[i] as its B operand
+        if (variant == 0) { ha[i] = (double)(i + 1); hb[i] = (double)((i * 7) % 13 - 6); }
+        else if (variant == 1) { ha[i] = 1.0; hb[i] = (double)i; }     // sums of B lanes per output
+        else { ha[i] = (double)i; hb[i] = 1.0; }                      // sums of A lanes per output
         fa[i] = (float)ha[i];
         fb[i] = (float)hb[i];
     }

@@ -335,18 +335,17 @@ void launch_batch_band(const void *arena, const long *off, const int *nts, const
 }
 
 // ------------------------------------------------------------ debug probes
-// One 16x16x4 MFMA per element type with A = a[i + 16k], B = b[k + 4j]; writes
-// the accumulator of every lane so the host can check the lane layout.
+// One 16x16x4 MFMA per element type; lane l supplies a[l] and b[l] and the
+// accumulator of every lane is written out so the host can read the lane layout.
 __global__ void mfma_probe_kernel(const double *a, const double *b, double *out64, const float *af,
                                   const float *bf, float *out32)
 {
     const int l = threadIdx.x;
-    const int i = l & 15, k = l >> 4;
     d4_t acc = vzero4<d4_t>();
-    acc = mfma16(a[i + 16 * k], b[k + 4 * i], acc);
+    acc = mfma16(a[l], b[l], acc);
     for (int m = 0; m < 4; ++m) out64[l * 4 + m] = acc[m];
     f4_t accf = vzero4<f4_t>();
-    accf = mfma16(af[i + 16 * k], bf[k + 4 * i], accf);
+    accf = mfma16(af[l], bf[l], accf);
     for (int m = 0; m < 4; ++m) out32[l * 4 + m] = accf[m];
 }
 

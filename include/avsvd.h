@@ -47,8 +47,10 @@ typedef enum { AVS_F64 = 0, AVS_F32 = 1 } avs_dtype;
 
 /* How the tile DAG is executed. */
 typedef enum {
-    AVS_SCHED_PERSISTENT = 0, /* one device-side dependency-counting persistent launch */
-    AVS_SCHED_HOST = 1        /* host-enqueued launches in the reference order (plain path) */
+    AVS_SCHED_PERSISTENT = 0, /* one device-side dependency-counting persistent launch (disabled:
+                                 returns AVS_ERR_UNSUPPORTED) */
+    AVS_SCHED_HOST = 1,       /* host-enqueued launches in the reference order (plain path) */
+    AVS_SCHED_STREAMS = 2     /* fused per-domain chain kernels, stream-ordered */
 } avs_sched;
 
 /* Reduction tree of each QR/LQ panel sweep.  FLAT is the reference's
@@ -78,7 +80,7 @@ typedef struct avs_opts {
 
 typedef struct avs_handle avs_handle;
 
-/* Default options (persistent schedule, autotuned tree). */
+/* Default options. */
 void avs_default_opts(avs_opts *opts);
 
 /* TiledMatrix(A, b, b) for an n x n matrix: allocates the tile-major working
@@ -221,12 +223,14 @@ avs_status avs_server_destroy(avs_server *server);
 
 /* ---- diagnostics ----
  * avs_debug_mfma_layout: runs one v_mfma_f64_16x16x4f64 and one
- * v_mfma_f32_16x16x4f32 with A[i][k] = i + 16k + 1 and B[k][j] = ((k + 4j)*7 % 13) - 6
- * and returns each lane's 4 accumulator values (out[lane*4 + m]), so a test
- * can pin the lane layout the update kernels rely on.
+ * v_mfma_f32_16x16x4f32 in which lane l supplies a_l as its A operand and b_l
+ * as its B operand (variant 0: a_l = l+1, b_l = (7l mod 13) - 6; variant 1:
+ * a_l = 1, b_l = l; variant 2: a_l = l, b_l = 1) and returns each lane's 4
+ * accumulator values (out[lane*4 + m]), so a test can pin the operand and
+ * accumulator lane layouts the update kernels rely on.
  * avs_schedule_stats: size of the task DAG and the list-scheduling makespan
  * estimate (microseconds) for an nt x nt tile grid, without a device. */
-avs_status avs_debug_mfma_layout(double *out64, float *out32);
+avs_status avs_debug_mfma_layout(int32_t variant, double *out64, float *out32);
 avs_status avs_schedule_stats(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
                               int32_t workers, int64_t *ntasks, double *est_us);
 /* The persistent executor's task list in dispatch order as a flat int32

@@ -37,15 +37,23 @@ def ptr(t):
 
 
 def test_mfma_lane_layout(gpu):
-    o64, o32 = avsvd.mfma_layout()
+    """Pins the 16x16x4 MFMA lane layout the update kernels assume:
+    A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, accumulator element m
+    of lane l = D[4*(l//16) + m][l % 16].  Prints the raw probes."""
+    import json
+    probes = {}
+    for v in (0, 1, 2):
+        o64, o32 = avsvd.mfma_layout(v)
+        probes[v] = (o64, o32)
+        print("MFMA_PROBE", v, json.dumps({"f64": o64.tolist(), "f32": o32.astype(float).tolist()}), flush=True)
     a = np.arange(1, 65, dtype=np.float64)
     bb = np.array([((i * 7) % 13) - 6 for i in range(64)], dtype=np.float64)
     A = np.array([[a[i + 16 * k] for k in range(4)] for i in range(16)])
-    B = np.array([[bb[k + 4 * j] for j in range(16)] for k in range(4)])
+    B = np.array([[bb[j + 16 * k] for j in range(16)] for k in range(4)])
     D = A @ B
     exp = np.array([[D[4 * (l // 16) + m, l % 16] for m in range(4)] for l in range(64)])
-    np.testing.assert_array_equal(o64, exp)
-    np.testing.assert_array_equal(o32.astype(np.float64), exp)
+    np.testing.assert_array_equal(probes[0][1].astype(np.float64), exp)
+    np.testing.assert_array_equal(probes[0][0], exp)
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])

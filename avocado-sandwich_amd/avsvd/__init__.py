@@ -22,13 +22,13 @@ from .batch import Batch, batch_svdvals
 from .server import Server
 from .tiled import (TiledMatrix, Blockbidiag_, tiled_QR_, OOC_QR_, OOC_Bidiag_, OOC_SVD_, compile_kernels,
                     bidiagonalize, bidiagonalize_band, diag_lapack, bdsdc_values, my_CU_svdval, svdvals,
-                    gaussian_matrix, geqrt_, tsqrt_, unmqr_, tsmqr_, schedule_stats, schedule_dump, mfma_layout)
+                    gaussian_matrix, geqrt_, tsqrt_, unmqr_, tsmqr_, schedule_stats, schedule_dump, phase_dump, mfma_layout)
 
 __all__ = [
     "Batch", "batch_svdvals", "Server",
     "TiledMatrix", "Blockbidiag_", "tiled_QR_", "OOC_QR_", "OOC_Bidiag_", "OOC_SVD_", "compile_kernels",
     "bidiagonalize", "bidiagonalize_band", "diag_lapack", "bdsdc_values", "my_CU_svdval", "svdvals",
-    "gaussian_matrix", "geqrt_", "tsqrt_", "unmqr_", "tsmqr_", "schedule_stats", "schedule_dump", "mfma_layout",
+    "gaussian_matrix", "geqrt_", "tsqrt_", "unmqr_", "tsmqr_", "schedule_stats", "schedule_dump", "phase_dump", "mfma_layout",
     "AvsError", "EXPORTS", "LIB_PATH", "lib", "AVS_F32", "AVS_F64", "SCHED_HOST", "SCHED_PERSISTENT", "SCHED_STREAMS",
     "TREE_FLAT", "TREE_HIER", "UPD_NATIVE", "UPD_SPLIT_BF16",
 ]

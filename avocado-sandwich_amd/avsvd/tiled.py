@@ -49,7 +49,7 @@ class TiledMatrix:
     aliases A: the tiled algorithms write their result back into A.
     `tiledim2d` is accepted for signature compatibility (the workgroup shape is
     fixed per tile size here).  Extra keyword options select the executor:
-    sched (SCHED_PERSISTENT / SCHED_HOST), tree (TREE_HIER / TREE_FLAT), domain.
+    sched (SCHED_STREAMS / SCHED_HOST), tree (TREE_HIER / TREE_FLAT), domain.
     """
 
     def __init__(self, A, blocksize_m, blocksize_n=None, tiledim2d=0, **opts):
@@ -269,6 +269,30 @@ def schedule_dump(nt, b, alg=0, kend=0, tree=1, domain=16, workers=256):
         i += 1 + ns
         out.append(t)
     return out
+
+
+def phase_dump(nt, b, alg=0, kend=0, tree=1, domain=16):
+    """Phases of the stream-ordered executor: list of (kind, ncols, [chain, ...]),
+    each chain a list of op dicts."""
+    need = ctypes.c_int64()
+    check(lib().avs_phase_dump(nt, b, alg, kend, tree, domain, None, 0, ctypes.byref(need)))
+    buf = np.zeros(max(need.value, 1), dtype=np.int32)
+    check(lib().avs_phase_dump(nt, b, alg, kend, tree, domain, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                               need.value, ctypes.byref(need)))
+    v = buf.tolist()
+    keys = ("kind", "mode", "trans", "vtile", "btile", "c1tile", "cstep", "fac", "ncols")
+    i, phases = 1, []
+    for _ in range(v[0]):
+        kind, ncols, nch = v[i], v[i + 1], v[i + 2]
+        i += 3
+        chains = []
+        for _ in range(nch):
+            nops = v[i]
+            i += 1
+            chains.append([dict(zip(keys, v[i + 9 * q:i + 9 * q + 9])) for q in range(nops)])
+            i += 9 * nops
+        phases.append((kind, ncols, chains))
+    return phases
 
 
 def mfma_layout(variant=0):

@@ -92,6 +92,18 @@ struct ProgKey {
 std::mutex g_prog_mu;
 std::map<ProgKey, std::shared_ptr<DevProgram>> g_progs;
 
+struct DevPhases {
+    PhaseProgram P;
+    ChainOp *dOps = nullptr;
+    int *dChain = nullptr;
+    ~DevPhases()
+    {
+        hipFree(dOps);
+        hipFree(dChain);
+    }
+};
+std::map<ProgKey, std::shared_ptr<DevPhases>> g_phases;
+
 } // namespace
 
 struct avs_handle {
@@ -112,6 +124,7 @@ struct avs_handle {
     int workers = 0, device = 0;
     long long wallclock_khz = 100000;
     std::shared_ptr<DevProgram> prog;
+    std::shared_ptr<DevPhases> phases;
 };
 
 namespace {
@@ -162,6 +175,32 @@ avs_status ensure_program(avs_handle *h, int alg, int kend)
     return AVS_OK;
 }
 
+avs_status ensure_phases(avs_handle *h, int alg, int kend)
+{
+    const int tree = resolve_tree(h), dom = resolve_domain(h);
+    const ProgKey key{h->device, h->nt, h->b, alg, kend, tree, dom, 0, 0};
+    if (h->phases && h->phases->P.alg == alg && h->phases->P.kend == kend && h->phases->P.tree == tree &&
+        h->phases->P.domain == dom)
+        return AVS_OK;
+    std::lock_guard<std::mutex> lk(g_prog_mu);
+    auto it = g_phases.find(key);
+    if (it != g_phases.end()) {
+        h->phases = it->second;
+        return AVS_OK;
+    }
+    auto dp = std::make_shared<DevPhases>();
+    build_phase_program(dp->P, h->nt, h->b, alg, kend, tree, dom);
+    const auto &ops = dp->P.ops;
+    const auto &cs = dp->P.chain_start;
+    HIPCHK(hipMalloc(&dp->dOps, sizeof(ChainOp) * std::max<size_t>(ops.size(), 1)));
+    HIPCHK(hipMalloc(&dp->dChain, sizeof(int) * std::max<size_t>(cs.size(), 1)));
+    if (!ops.empty()) HIPCHK(hipMemcpy(dp->dOps, ops.data(), sizeof(ChainOp) * ops.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dp->dChain, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice));
+    g_phases[key] = dp;
+    h->phases = dp;
+    return AVS_OK;
+}
+
 int split_of(const avs_handle *h) { return h->dtype == AVS_F32 && h->opts.update_mode == AVS_UPD_SPLIT_BF16; }
 
 template <typename TS, int B> avs_status run_host_enqueued(avs_handle *h)
@@ -196,6 +235,23 @@ template <typename TS, int B> avs_status run_host_enqueued(avs_handle *h)
     return AVS_OK;
 }
 
+template <typename TS, int B> avs_status run_streams(avs_handle *h)
+{
+    const DevPhases &D = *h->phases;
+    const int per = upd_tiles_for(B);
+    for (const Phase &ph : D.P.phases) {
+        if (ph.kind == TK_PANEL) {
+            launch_chain_panel<TS, B>(ph.nchains, h->dTiles, h->dTau, h->dTjj, D.dOps, D.dChain, ph.chain0, h->stream);
+        } else {
+            const int groups = (ph.ncols + per - 1) / per;
+            launch_chain_update<TS, B>(ph.nchains, groups, per, h->dTiles, h->dTjj, D.dOps, D.dChain, ph.chain0,
+                                       split_of(h), h->stream);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return AVS_OK;
+}
+
 template <typename TS, int B> avs_status run_persistent(avs_handle *h)
 {
     const Program &P = h->prog->P;
@@ -220,13 +276,14 @@ avs_status run_alg(avs_handle *h, int alg, int32_t kend)
                     "and its cause is not yet identified; use AVS_SCHED_STREAMS or AVS_SCHED_HOST");
     if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
     if (kend <= 0 || kend > h->nt) kend = h->nt;
-    avs_status st = ensure_program(h, alg, kend);
+    avs_status st = h->opts.sched == AVS_SCHED_STREAMS ? ensure_phases(h, alg, kend) : ensure_program(h, alg, kend);
     if (st) return st;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     st = dispatch_tb(h->dtype, h->b, [&](auto tp, auto bc) -> avs_status {
         typedef std::remove_pointer_t<decltype(tp)> TS;
         constexpr int B = decltype(bc)::value;
         if (h->opts.sched == AVS_SCHED_PERSISTENT) return run_persistent<TS, B>(h);
+        if (h->opts.sched == AVS_SCHED_STREAMS) return run_streams<TS, B>(h);
         return run_host_enqueued<TS, B>(h);
     });
     if (st) return st;
@@ -257,7 +314,7 @@ void avs_default_opts(avs_opts *o)
 {
     if (!o) return;
     memset(o, 0, sizeof *o);
-    o->sched = AVS_SCHED_HOST;
+    o->sched = AVS_SCHED_STREAMS;
     o->tree = AVS_TREE_HIER;
     o->domain = 0;
     o->update_mode = AVS_UPD_NATIVE;
@@ -589,6 +646,34 @@ avs_status avs_schedule_stats(int32_t nt, int32_t b, int32_t alg, int32_t kend, 
     build_program(P, nt, b, alg, kend <= 0 ? nt : kend, tree, domain, upd_tiles_for(b), workers);
     if (ntasks) *ntasks = (int64_t)P.prog.size();
     if (est_us) *est_us = P.est_makespan_us;
+    return AVS_OK;
+}
+
+avs_status avs_phase_dump(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
+                          int32_t *out, int64_t cap, int64_t *needed)
+{
+    if (nt <= 0) return fail(-1, "nt must be positive");
+    if (b <= 0) return fail(-2, "b must be positive");
+    if (!needed) return fail(-9, "null needed");
+    PhaseProgram P;
+    build_phase_program(P, nt, b, alg, kend <= 0 ? nt : kend, tree, domain);
+    std::vector<int32_t> v;
+    v.push_back((int32_t)P.phases.size());
+    for (const Phase &ph : P.phases) {
+        v.push_back(ph.kind);
+        v.push_back(ph.ncols);
+        v.push_back(ph.nchains);
+        for (int c = ph.chain0; c < ph.chain0 + ph.nchains; ++c) {
+            v.push_back(P.chain_start[c + 1] - P.chain_start[c]);
+            for (int o = P.chain_start[c]; o < P.chain_start[c + 1]; ++o) {
+                const ChainOp &t = P.ops[o];
+                const int32_t f[] = {t.kind, t.mode, t.trans, t.vtile, t.btile, t.c1tile, t.cstep, t.fac, t.ncols};
+                v.insert(v.end(), f, f + 9);
+            }
+        }
+    }
+    *needed = (int64_t)v.size();
+    if (out && cap >= (int64_t)v.size()) memcpy(out, v.data(), sizeof(int32_t) * v.size());
     return AVS_OK;
 }
 

@@ -302,8 +302,12 @@ __device__ void tjj_body(const TS *__restrict__ V, long ldv, bool vtr, const dou
 //                               accumulators feed the B operand directly)
 //   W' = T_JJ^T W              (4 MFMA)
 //   C1[J] -= W';  C2 -= Y_J W' (B/16 x 4 MFMA)
-// Accumulator layout of v_mfma_*_16x16x4: lane l, element m holds row
-// 4*(l/16)+m, column l%16 of the 16x16 block.
+// Operand layout of the 16x16x4 MFMAs: A[i][k] from lane i+16k, B[k][j] from
+// lane j+16k.  Accumulator element m of lane l holds column l%16 and row
+// 4*(l/16)+m for f32 but row (l/16)+4*m for f64 (measured on gfx950); AccRow
+// gives row = base*(l/16) + step*m.  The K index of a product only has to agree
+// between the A and B operands, so every K loop below walks C rows in the
+// accumulator's own order.
 // C1 tile k sits at C1 + k*s1 (elements), C2 tile k at C2 + k*s2.
 // ---------------------------------------------------------------------------
 template <typename V4> __device__ __forceinline__ V4 vzero4()
@@ -312,6 +316,10 @@ template <typename V4> __device__ __forceinline__ V4 vzero4()
     v[0] = 0; v[1] = 0; v[2] = 0; v[3] = 0;
     return v;
 }
+
+template <typename TS> struct AccRow;
+template <> struct AccRow<float> { static constexpr int base = 4, step = 1; };
+template <> struct AccRow<double> { static constexpr int base = 1, step = 4; };
 
 template <typename TS, int B, int MODE, bool APPLYT>
 __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tjj,
@@ -336,6 +344,7 @@ __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const 
     __syncthreads();
 
     const int lane = t & 63, w = t >> 6, q = lane >> 4, il = lane & 15;
+    constexpr int RB = AccRow<TS>::base, RS = AccRow<TS>::step;
     // logical element (row, col) of a C tile is at base + row*rs + col*cs
     const long rs1 = ctr ? ld1 : 1, cs1 = ctr ? 1 : ld1;
     const long rs2 = ctr ? ld2 : 1, cs2 = ctr ? 1 : ld2;
@@ -343,20 +352,20 @@ __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const 
     for (int sidx = w; sidx < nstrips; sidx += NWAVES) {
         const int tile = sidx / NB;
         const int col = (sidx % NB) * 16 + il;
-        // this lane's column of each tile, starting at row 4*q
-        TS *c1 = C1 + (long)tile * s1 + (long)col * cs1 + (long)(4 * q) * rs1;
-        TS *c2 = C2 + (long)tile * s2 + (long)col * cs2 + (long)(4 * q) * rs2;
+        // this lane's column of each tile, starting at its first accumulator row
+        TS *c1 = C1 + (long)tile * s1 + (long)col * cs1 + (long)(RB * q) * rs1;
+        TS *c2 = C2 + (long)tile * s2 + (long)col * cs2 + (long)(RB * q) * rs2;
         acc_t acc2[NB];
 #pragma unroll
         for (int tb = 0; tb < NB; ++tb) {
 #pragma unroll
-            for (int m = 0; m < 4; ++m) acc2[tb][m] = c2[(long)(16 * tb + m) * rs2];
+            for (int m = 0; m < 4; ++m) acc2[tb][m] = c2[(long)(16 * tb + RS * m) * rs2];
         }
         acc_t c1n;
         if (MODE != MODE_GE) {
             const int J0 = APPLYT ? 0 : NB - 1;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) c1n[m] = c1[(long)(16 * J0 + m) * rs1];
+            for (int m = 0; m < 4; ++m) c1n[m] = c1[(long)(16 * J0 + RS * m) * rs1];
         }
 #pragma unroll 1
         for (int jj = 0; jj < NB; ++jj) {
@@ -367,20 +376,20 @@ __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const 
                 if (jj + 1 < NB) {
                     const int Jn = APPLYT ? jj + 1 : (NB - 2 - jj);
 #pragma unroll
-                    for (int m = 0; m < 4; ++m) c1n[m] = c1[(long)(16 * Jn + m) * rs1];
+                    for (int m = 0; m < 4; ++m) c1n[m] = c1[(long)(16 * Jn + RS * m) * rs1];
                 }
                 Wa = c1v;
             } else {
                 Wa = vzero4<acc_t>();
             }
-            // W = C1[J] + Y_J^T C2   (K order: lane group q supplies rows 16tb+4q+m)
+            // W = C1[J] + Y_J^T C2   (K order: lane group q supplies rows 16tb+RB*q+RS*m)
 #pragma unroll
             for (int tb = 0; tb < NB; ++tb) {
                 if (MODE == MODE_GE && tb < J) continue;
                 if (MODE == MODE_TT && tb > J) continue;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const TS av = YL[(16 * tb + 4 * q + m) * S + 16 * J + il];
+                    const TS av = YL[(16 * tb + RB * q + RS * m) * S + 16 * J + il];
                     Wa = mfma16(av, acc2[tb][m], Wa);
                 }
             }
@@ -388,12 +397,12 @@ __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const 
             acc_t Wp = vzero4<acc_t>();
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                const TS av = APPLYT ? TL[(16 * J + 4 * q + m) * 16 + il] : TL[(16 * J + il) * 16 + 4 * q + m];
+                const TS av = APPLYT ? TL[(16 * J + RB * q + RS * m) * 16 + il] : TL[(16 * J + il) * 16 + RB * q + RS * m];
                 Wp = mfma16(av, Wa[m], Wp);
             }
             if (MODE != MODE_GE) {
 #pragma unroll
-                for (int m = 0; m < 4; ++m) c1[(long)(16 * J + m) * rs1] = c1v[m] - Wp[m];
+                for (int m = 0; m < 4; ++m) c1[(long)(16 * J + RS * m) * rs1] = c1v[m] - Wp[m];
             }
             // C2 -= Y_J W'
 #pragma unroll
@@ -402,7 +411,7 @@ __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const 
                 if (MODE == MODE_TT && tb > J) continue;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const TS av = -YL[(16 * tb + il) * S + 16 * J + 4 * q + m];
+                    const TS av = -YL[(16 * tb + il) * S + 16 * J + RB * q + RS * m];
                     acc2[tb] = mfma16(av, Wp[m], acc2[tb]);
                 }
             }
@@ -410,7 +419,7 @@ __device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const 
 #pragma unroll
         for (int tb = 0; tb < NB; ++tb) {
 #pragma unroll
-            for (int m = 0; m < 4; ++m) c2[(long)(16 * tb + m) * rs2] = acc2[tb][m];
+            for (int m = 0; m < 4; ++m) c2[(long)(16 * tb + RS * m) * rs2] = acc2[tb][m];
         }
     }
 }

@@ -189,6 +189,80 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
     }
 }
 
+// ------------------------------------------------ stream-ordered executor
+// Panel phase: workgroup c runs the panel operations of chain c in order
+// (a domain's GEQRT + TSQRT chain, or one tree merge).
+template <typename TS, int B>
+__global__ void __launch_bounds__(NTHREADS) chain_panel_kernel(TS *tiles, double *taus, double *tjjs,
+                                                               const ChainOp *__restrict__ ops,
+                                                               const int *__restrict__ chain_start, int chain0)
+{
+    __shared__ double smem[Lds<TS, B>::panel];
+    const long tsz = (long)B * B;
+    const int c = chain0 + blockIdx.x;
+    for (int o = chain_start[c]; o < chain_start[c + 1]; ++o) {
+        const ChainOp T = ops[o];
+        TS *top = tiles + (long)T.vtile * tsz;
+        TS *bot = tiles + (long)T.btile * tsz;
+        double *tau = taus + (long)T.fac * B;
+        double *tjj = tjjs + (long)T.fac * 16 * B;
+        const bool tr = T.trans != 0;
+        if (T.mode == MODE_GE) panel_body<TS, B, MODE_GE>(top, B, top, B, tr, tau, tjj, smem);
+        else if (T.mode == MODE_TS) panel_body<TS, B, MODE_TS>(top, B, bot, B, tr, tau, tjj, smem);
+        else panel_body<TS, B, MODE_TT>(top, B, bot, B, tr, tau, tjj, smem);
+        __syncthreads();
+    }
+}
+
+// Update phase: workgroup (chain c, column group g) applies every operation of
+// chain c, in order, to the `per` column tiles of group g.  The chain's C1 row
+// tile is rewritten by each operation and stays hot in L2.
+template <typename TS, int B>
+__global__ void __launch_bounds__(NTHREADS) chain_update_kernel(TS *tiles, const double *tjjs,
+                                                                const ChainOp *__restrict__ ops,
+                                                                const int *__restrict__ chain_start, int chain0,
+                                                                int groups, int per, int split)
+{
+    __shared__ double smem[Lds<TS, B>::upd];
+    const long tsz = (long)B * B;
+    const int c = chain0 + blockIdx.x / groups;
+    const int g = blockIdx.x % groups;
+    for (int o = chain_start[c]; o < chain_start[c + 1]; ++o) {
+        const ChainOp T = ops[o];
+        const int first = g * per;
+        const int cnt = min(per, T.ncols - first);
+        if (cnt <= 0) break;
+        const TS *V = tiles + (long)T.vtile * tsz;
+        const long st = (long)T.cstep * tsz;
+        TS *C2 = tiles + (long)T.btile * tsz + (long)first * st;
+        TS *C1 = tiles + (long)T.c1tile * tsz + (long)first * st;
+        const double *tjj = tjjs + (long)T.fac * 16 * B;
+        const bool tr = T.trans != 0;
+        if (T.mode == MODE_GE)
+            update_sel<TS, B, MODE_GE>(split != 0, V, B, tr, tjj, C2, B, st, C2, B, st, tr, cnt, smem);
+        else if (T.mode == MODE_TS)
+            update_sel<TS, B, MODE_TS>(split != 0, V, B, tr, tjj, C1, B, st, C2, B, st, tr, cnt, smem);
+        else
+            update_sel<TS, B, MODE_TT>(split != 0, V, B, tr, tjj, C1, B, st, C2, B, st, tr, cnt, smem);
+        __syncthreads();
+    }
+}
+
+template <typename TS, int B>
+void launch_chain_panel(int nchains, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
+                        const int *chain_start, int chain0, hipStream_t s)
+{
+    hipLaunchKernelGGL((chain_panel_kernel<TS, B>), dim3(nchains), dim3(NTHREADS), 0, s, (TS *)tiles, taus, tjjs, ops,
+                       chain_start, chain0);
+}
+template <typename TS, int B>
+void launch_chain_update(int nchains, int groups, int per, void *tiles, const double *tjjs, const ChainOp *ops,
+                         const int *chain_start, int chain0, int split, hipStream_t s)
+{
+    hipLaunchKernelGGL((chain_update_kernel<TS, B>), dim3(nchains * groups), dim3(NTHREADS), 0, s, (TS *)tiles, tjjs,
+                       ops, chain_start, chain0, groups, per, split);
+}
+
 // ---------------------------------------------------------- batched form
 // One grouped persistent launch for many independent matrices (possibly of
 // different sizes, all with tile size B).  Each workgroup takes whole matrices
@@ -458,7 +532,11 @@ void launch_mfma_probe(const double *a, const double *b, double *o64, const floa
     template int persistent_occupancy<TS, B>();                                                                        \
     template void launch_batch<TS, B>(int, void *, const long *, const int *, const int *, int, double *, int *, int, \
                                       hipStream_t);                                                                    \
-    template int batch_occupancy<TS, B>();
+    template int batch_occupancy<TS, B>();                                                                             \
+    template void launch_chain_panel<TS, B>(int, void *, double *, double *, const ChainOp *, const int *, int,        \
+                                            hipStream_t);                                                              \
+    template void launch_chain_update<TS, B>(int, int, int, void *, const double *, const ChainOp *, const int *, int, \
+                                             int, hipStream_t);
 
 AVS_INST_B(double, 32)
 AVS_INST_B(double, 64)

@@ -6,6 +6,7 @@
 namespace avs {
 
 struct Task;
+struct ChainOp;
 
 template <typename TS> void launch_pack(const void *A, long lda, void *tiles, int nt, int b, hipStream_t s);
 template <typename TS>
@@ -37,6 +38,13 @@ void launch_batch_pack(const void *stage, const long *soff, void *arena, const l
 template <typename TS>
 void launch_batch_band(const void *arena, const long *off, const int *nts, const long *col0, long total_cols,
                        double *bands, const long *boff, int count, int b, hipStream_t s);
+
+template <typename TS, int B>
+void launch_chain_panel(int nchains, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
+                        const int *chain_start, int chain0, hipStream_t s);
+template <typename TS, int B>
+void launch_chain_update(int nchains, int groups, int per, void *tiles, const double *tjjs, const ChainOp *ops,
+                         const int *chain_start, int chain0, int split, hipStream_t s);
 
 void launch_mfma_probe(const double *a, const double *b, double *o64, const float *af, const float *bf, float *o32,
                        hipStream_t s);

@@ -245,4 +245,119 @@ void build_program(Program &P, int nt, int B, int alg, int kend, int tree, int d
     P.est_makespan_us = now;
 }
 
+namespace {
+
+struct PhaseBuilder {
+    PhaseProgram &P;
+    int nt;
+    explicit PhaseBuilder(PhaseProgram &p, int nt_) : P(p), nt(nt_) {}
+    int tid(int i, int j) const { return j * nt + i; }
+    // opens a phase; chains are appended with chain()
+    void begin(int kind, int ncols)
+    {
+        Phase ph{kind, (int)P.chain_start.size() - 1, 0, ncols};
+        P.phases.push_back(ph);
+    }
+    void chain(const std::vector<ChainOp> &ops)
+    {
+        for (auto &o : ops) P.ops.push_back(o);
+        P.chain_start.push_back((int)P.ops.size());
+        P.phases.back().nchains++;
+    }
+    ChainOp panel(int mode, int trans, int top, int bot, int fac)
+    {
+        ChainOp o{};
+        o.kind = TK_PANEL;
+        o.mode = (int8_t)mode;
+        o.trans = (int8_t)trans;
+        o.vtile = top;
+        o.btile = bot;
+        o.fac = fac;
+        return o;
+    }
+    ChainOp update(int mode, int trans, int vt, int c1first, int c2first, int step, int cnt, int fac)
+    {
+        ChainOp o{};
+        o.kind = TK_UPDATE;
+        o.mode = (int8_t)mode;
+        o.trans = (int8_t)trans;
+        o.vtile = vt;
+        o.btile = c2first;
+        o.c1tile = mode == MODE_GE ? c2first : c1first;
+        o.cstep = step;
+        o.fac = fac;
+        o.ncols = cnt;
+        return o;
+    }
+    void sweep(int k, bool lq, int tree, int domain)
+    {
+        const int m = lq ? nt - k - 1 : nt - k;
+        if (m <= 0) return;
+        auto T = [&](int p) { return lq ? tid(k, k + 1 + p) : tid(k + p, k); };
+        auto U0 = [&](int p) { return lq ? tid(k + 1, k + 1 + p) : tid(k + p, k + 1); };
+        const int step = lq ? 1 : nt;
+        const int cnt = nt - k - 1;
+        const int nt2 = nt * nt;
+        const int famGE = lq ? 2 : 0, famTT = lq ? 3 : 1;
+        const int h = (tree == 0 || domain <= 0) ? m : domain;
+        std::vector<int> heads;
+        for (int d0 = 0; d0 < m; d0 += h) heads.push_back(d0);
+        // domain panel chains
+        begin(TK_PANEL, 0);
+        for (int d0 : heads) {
+            std::vector<ChainOp> c;
+            const int d1 = std::min(d0 + h, m);
+            c.push_back(panel(MODE_GE, lq, T(d0), T(d0), famGE * nt2 + T(d0)));
+            for (int p = d0 + 1; p < d1; ++p) c.push_back(panel(MODE_TS, lq, T(d0), T(p), famGE * nt2 + T(p)));
+            chain(c);
+        }
+        if (cnt > 0) {
+            begin(TK_UPDATE, cnt);
+            for (int d0 : heads) {
+                std::vector<ChainOp> c;
+                const int d1 = std::min(d0 + h, m);
+                c.push_back(update(MODE_GE, lq, T(d0), U0(d0), U0(d0), step, cnt, famGE * nt2 + T(d0)));
+                for (int p = d0 + 1; p < d1; ++p)
+                    c.push_back(update(MODE_TS, lq, T(p), U0(d0), U0(p), step, cnt, famGE * nt2 + T(p)));
+                chain(c);
+            }
+        }
+        // pairwise merges of the domain heads, one level per phase pair
+        const int nh = (int)heads.size();
+        for (int s2 = 1; s2 < nh; s2 *= 2) {
+            std::vector<std::pair<int, int>> pairs;
+            for (int a = 0; a + s2 < nh; a += 2 * s2) pairs.push_back({heads[a], heads[a + s2]});
+            begin(TK_PANEL, 0);
+            for (auto &pr : pairs) chain({panel(MODE_TT, lq, T(pr.first), T(pr.second), famTT * nt2 + T(pr.second))});
+            if (cnt > 0) {
+                begin(TK_UPDATE, cnt);
+                for (auto &pr : pairs)
+                    chain({update(MODE_TT, lq, T(pr.second), U0(pr.first), U0(pr.second), step, cnt,
+                                  famTT * nt2 + T(pr.second))});
+            }
+        }
+    }
+};
+
+} // namespace
+
+void build_phase_program(PhaseProgram &P, int nt, int B, int alg, int kend, int tree, int domain)
+{
+    P = PhaseProgram();
+    P.nt = nt;
+    P.B = B;
+    P.alg = alg;
+    P.kend = kend;
+    P.tree = tree;
+    P.domain = domain;
+    P.chain_start.push_back(0);
+    PhaseBuilder pb(P, nt);
+    for (int k = 0; k < kend && k < nt; ++k) {
+        pb.sweep(k, false, tree, domain);
+        if (alg == ALG_TILED_QR) continue;
+        if (k == nt - 1) break;
+        pb.sweep(k, true, tree, domain);
+    }
+}
+
 } // namespace avs

@@ -18,6 +18,23 @@ struct Program {
     long n_panel = 0, n_update = 0;
 };
 
+// Stream-ordered form of the same reduction: a sequence of phases, each one
+// kernel launch whose workgroups run independent chains of operations
+// (a chain runs sequentially inside one workgroup).  Panel phases: one
+// workgroup per chain.  Update phases: one workgroup per (chain, column group).
+struct Phase {
+    int kind;         // TK_PANEL or TK_UPDATE
+    int chain0, nchains;
+    int ncols;        // update: column tiles of every op in the phase
+};
+struct PhaseProgram {
+    int nt = 0, B = 0, alg = 0, kend = 0, tree = 0, domain = 0;
+    std::vector<ChainOp> ops;
+    std::vector<int> chain_start; // ops of chain c: [chain_start[c], chain_start[c+1])
+    std::vector<Phase> phases;
+};
+void build_phase_program(PhaseProgram &P, int nt, int B, int alg, int kend, int tree, int domain);
+
 // Builds the task DAG of Blockbidiag! (tiledalgos.jl:17-34) or tiled_QR!
 // (tiledalgos.jl:6-15) over an nt x nt tile grid, for kend steps.
 // tree: 0 = flat (reference), 1 = hierarchical with `domain` tiles per domain.

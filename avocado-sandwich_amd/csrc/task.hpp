@@ -30,4 +30,13 @@ struct Task {
     int32_t sig0, nsig; // slice of the signal array (tiles written)
 };
 
+// One operation of a chain executed by the stream-ordered executor.  Panel
+// ops: vtile = top tile, btile = bottom tile.  Update ops: vtile = reflector
+// tile; btile / c1tile = first C2 / C1 tile of the row being updated, cstep =
+// tile-index step between consecutive column tiles, ncols = column tiles.
+struct ChainOp {
+    int8_t kind, mode, trans, pad;
+    int32_t vtile, btile, c1tile, cstep, fac, ncols;
+};
+
 } // namespace avs

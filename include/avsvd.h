@@ -240,6 +240,11 @@ avs_status avs_schedule_stats(int32_t nt, int32_t b, int32_t alg, int32_t kend, 
  * Lets a host test replay the DAG exactly as the device executes it. */
 avs_status avs_schedule_dump(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
                              int32_t workers, int32_t *out, int64_t cap, int64_t *needed);
+/* The stream-ordered executor's phases as a flat int32 stream: nphases, then
+ * per phase kind, ncols, nchains and per chain nops followed by nops x (kind,
+ * mode, trans, vtile, btile, c1tile, cstep, fac, ncols). */
+avs_status avs_phase_dump(int32_t nt, int32_t b, int32_t alg, int32_t kend, int32_t tree, int32_t domain,
+                          int32_t *out, int64_t cap, int64_t *needed);
 
 #ifdef __cplusplus
 }

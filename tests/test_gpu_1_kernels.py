@@ -38,8 +38,9 @@ def ptr(t):
 
 def test_mfma_lane_layout(gpu):
     """Pins the 16x16x4 MFMA lane layout the update kernels assume:
-    A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, accumulator element m
-    of lane l = D[4*(l//16) + m][l % 16].  Prints the raw probes."""
+    A[i][k] from lane i + 16k, B[k][j] from lane j + 16k; accumulator element m
+    of lane l = D[4*(l//16) + m][l % 16] for f32 and D[l//16 + 4*m][l % 16] for
+    f64.  Variants 1 and 2 (all-ones operands) isolate the B and A lane maps."""
     import json
     probes = {}
     for v in (0, 1, 2):
@@ -51,9 +52,20 @@ def test_mfma_lane_layout(gpu):
     A = np.array([[a[i + 16 * k] for k in range(4)] for i in range(16)])
     B = np.array([[bb[j + 16 * k] for j in range(16)] for k in range(4)])
     D = A @ B
-    exp = np.array([[D[4 * (l // 16) + m, l % 16] for m in range(4)] for l in range(64)])
-    np.testing.assert_array_equal(probes[0][1].astype(np.float64), exp)
-    np.testing.assert_array_equal(probes[0][0], exp)
+    e32 = np.array([[D[4 * (l // 16) + m, l % 16] for m in range(4)] for l in range(64)])
+    e64 = np.array([[D[l // 16 + 4 * m, l % 16] for m in range(4)] for l in range(64)])
+    np.testing.assert_array_equal(probes[0][1].astype(np.float64), e32)
+    np.testing.assert_array_equal(probes[0][0], e64)
+    # variant 1 (A = 1, B = lane): every output in column j is sum_k (j + 16k)
+    for v, (o64, o32) in ((1, probes[1]),):
+        col = np.array([[4 * (l % 16) + 96 for _ in range(4)] for l in range(64)], dtype=np.float64)
+        np.testing.assert_array_equal(o64, col)
+        np.testing.assert_array_equal(o32.astype(np.float64), col)
+    # variant 2 (A = lane, B = 1): output row i is sum_k (i + 16k) = 4i + 96
+    o64, o32 = probes[2]
+    np.testing.assert_array_equal(o64, np.array([[4 * (l // 16 + 4 * m) + 96 for m in range(4)] for l in range(64)]))
+    np.testing.assert_array_equal(o32.astype(np.float64),
+                                  np.array([[4 * (4 * (l // 16) + m) + 96 for m in range(4)] for l in range(64)]))
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])

@@ -2,9 +2,9 @@
 
 Mirrors KAbasedSVD/test/test_tiledalgo.jl (sigma(band) = sigma(A), R of the
 tiled QR = LAPACK R up to signs) and CUDAbasedSVD/test.jl (svdvals through
-band + bulge chasing + bdsdc), for both executors (persistent DAG kernel and
-host-enqueued launches), both trees (the reference's flat tree and the
-hierarchical one), Float64 and Float32.  Bars: sigma error vs FP64 LAPACK
+band + bulge chasing + bdsdc), for both executors (stream-ordered phase
+launches and host-enqueued per-task launches), both trees (the reference's
+flat tree and the hierarchical one), Float64 and Float32.  Bars: sigma error vs FP64 LAPACK
 <= 10*n*eps of the working precision; band structure exact.
 """
 import os
@@ -37,9 +37,14 @@ CASES = [
     # (n, b, dtype, sched, tree, domain)
     (64, 32, np.float64, avsvd.SCHED_HOST, avsvd.TREE_FLAT, 0),
     (256, 64, np.float64, avsvd.SCHED_HOST, avsvd.TREE_HIER, 2),
-    (512, 64, np.float32, avsvd.SCHED_HOST, avsvd.TREE_FLAT, 0),
-    (512, 128, np.float64, avsvd.SCHED_HOST, avsvd.TREE_HIER, 2),
     (256, 32, np.float32, avsvd.SCHED_HOST, avsvd.TREE_HIER, 2),
+    (64, 32, np.float64, avsvd.SCHED_STREAMS, avsvd.TREE_FLAT, 0),
+    (256, 32, np.float64, avsvd.SCHED_STREAMS, avsvd.TREE_FLAT, 0),
+    (256, 64, np.float64, avsvd.SCHED_STREAMS, avsvd.TREE_HIER, 2),
+    (512, 64, np.float32, avsvd.SCHED_STREAMS, avsvd.TREE_FLAT, 0),
+    (512, 128, np.float64, avsvd.SCHED_STREAMS, avsvd.TREE_HIER, 2),
+    (1024, 128, np.float64, avsvd.SCHED_STREAMS, avsvd.TREE_HIER, 4),
+    (1024, 64, np.float32, avsvd.SCHED_STREAMS, avsvd.TREE_HIER, 3),
 ]
 
 PERSISTENT_OFF = ("persistent dependency-counting executor disabled: its launch did not complete on the "
@@ -84,18 +89,25 @@ def test_blockbidiag_sigma(gpu, golden_dir, n, b, dt, sched, tree, domain):
 @pytest.mark.parametrize("n,b", [(256, 32), (512, 128)])
 def test_flat_tree_band_matches_oracle(gpu, n, b):
     """The reference schedule on the device gives the reference's band (same
-    reflectors; only the compact-WY arithmetic order differs)."""
+    reflectors; only the compact-WY arithmetic order differs), and both
+    executors give the same bits (same tile kernels in the same per-tile order)."""
     a = O.gaussian_matrix(n, BASE + 3 * n)
     ref = O.blockbidiag(a, b)
-    for sched in (avsvd.SCHED_HOST,):
+    outs = []
+    for sched in (avsvd.SCHED_HOST, avsvd.SCHED_STREAMS):
+        print("case n=%d b=%d sched=%d" % (n, b, sched), flush=True)
         A = np.asfortranarray(a.copy())
         T = avsvd.TiledMatrix(A, b, b, sched=sched, tree=avsvd.TREE_FLAT, wait_ms=10000)
         avsvd.Blockbidiag_(T)
         np.testing.assert_allclose(T.A, ref, rtol=0, atol=1e-11 * np.abs(a).max() * np.sqrt(n))
+        outs.append(T.A.copy())
+    np.testing.assert_array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("sched,tree", [(avsvd.SCHED_HOST, avsvd.TREE_FLAT),
-                                        (avsvd.SCHED_HOST, avsvd.TREE_HIER)])
+                                        (avsvd.SCHED_HOST, avsvd.TREE_HIER),
+                                        (avsvd.SCHED_STREAMS, avsvd.TREE_FLAT),
+                                        (avsvd.SCHED_STREAMS, avsvd.TREE_HIER)])
 def test_tiled_qr_R(gpu, sched, tree):
     n, b = 384, 64
     a = O.gaussian_matrix(n, BASE + 9 * n)
@@ -107,12 +119,13 @@ def test_tiled_qr_R(gpu, sched, tree):
     np.testing.assert_allclose(out * np.sign(np.diag(out))[:, None], r * np.sign(np.diag(r))[:, None], atol=1e-10)
 
 
-def test_kend_partial_matches_oracle(gpu):
+@pytest.mark.parametrize("sched", [avsvd.SCHED_HOST, avsvd.SCHED_STREAMS])
+def test_kend_partial_matches_oracle(gpu, sched):
     n, b, kend = 256, 32, 3
     a = O.gaussian_matrix(n, BASE + 77)
     ref = O.blockbidiag(a, b, kend=kend)
     A = np.asfortranarray(a.copy())
-    T = avsvd.TiledMatrix(A, b, b, sched=avsvd.SCHED_HOST, tree=avsvd.TREE_FLAT)
+    T = avsvd.TiledMatrix(A, b, b, sched=sched, tree=avsvd.TREE_FLAT)
     avsvd.Blockbidiag_(T, kend=kend)
     np.testing.assert_allclose(T.A, ref, atol=1e-11 * np.sqrt(n))
 
@@ -134,7 +147,7 @@ def test_n4096_fp64_b128_against_cached_gesdd(gpu, golden_dir):
     n, b = 4096, 128
     a = avsvd.gaussian_matrix(n, BASE)
     fro = np.linalg.norm(a)
-    T = avsvd.TiledMatrix(a, b, b, sched=avsvd.SCHED_HOST, tree=avsvd.TREE_HIER)
+    T = avsvd.TiledMatrix(a, b, b, sched=avsvd.SCHED_STREAMS, tree=avsvd.TREE_HIER)
     avsvd.Blockbidiag_(T, sync=False)
     bnd = T.band()
     assert abs(np.linalg.norm(bnd) - fro) <= 1e-12 * fro

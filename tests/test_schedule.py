@@ -163,3 +163,61 @@ def test_schedule_stats_hier_shorter_than_flat():
     nf, ef = avsvd.schedule_stats(64, 128, tree=0, workers=256)
     nh, eh = avsvd.schedule_stats(64, 128, tree=1, domain=8, workers=256)
     assert nh > nf and eh < ef
+
+
+def replay_phases(a, b, alg, tree, domain, seed):
+    """Run the stream-ordered executor's phases with the oracle's tile
+    kernels: phases in order, chains of a phase in random order, the ops of a
+    chain in sequence, each update op over all its column tiles."""
+    S = TileState(a, b)
+    rng = random.Random(seed)
+    phases = avsvd.phase_dump(S.nt, b, alg=alg, tree=tree, domain=domain)
+    for kind, ncols, chains in phases:
+        order = list(range(len(chains)))
+        rng.shuffle(order)
+        for c in order:
+            for op in chains[c]:
+                t = dict(op)
+                if kind == 1:
+                    t["ntiles"] = t["ncols"]
+                run_task(S, t)
+    return S, phases
+
+
+@pytest.mark.parametrize("b,nt", [(4, 5), (8, 4)])
+def test_flat_phases_match_oracle_bitwise(b, nt):
+    n = b * nt
+    a = O.gaussian_matrix(n, 20261203 + 31 * n)
+    S, _ = replay_phases(a, b, alg=0, tree=0, domain=0, seed=3)
+    np.testing.assert_array_equal(S.dense(True), O.blockbidiag(a, b))
+
+
+@pytest.mark.parametrize("b,nt,domain", [(4, 9, 2), (4, 12, 3), (8, 7, 2), (4, 16, 4)])
+def test_hierarchical_phases_keep_sigma(b, nt, domain):
+    n = b * nt
+    a = O.gaussian_matrix(n, 20261203 + 17 * n + domain)
+    S, phases = replay_phases(a, b, alg=0, tree=1, domain=domain, seed=domain)
+    assert O.sigma_err(sl.svdvals(S.dense(True)), sl.svdvals(a)) < 10 * n * np.finfo(float).eps
+    # chains of one phase touch disjoint tiles (they run concurrently on the device)
+    for kind, ncols, chains in phases:
+        seen = set()
+        for ch in chains:
+            mine = set()
+            for op in ch:
+                if kind == 0:
+                    mine |= {op["btile"]} if op["mode"] != 0 else set()
+                else:
+                    for q in range(op["ncols"]):
+                        mine.add(op["btile"] + q * op["cstep"])
+                        if op["mode"] != 0:
+                            mine.add(op["c1tile"] + q * op["cstep"])
+            assert not (mine & seen)
+            seen |= mine
+
+
+def test_tiled_qr_phases():
+    b, nt = 4, 6
+    n = b * nt
+    a = O.gaussian_matrix(n, 20261203 + 5 * n)
+    S, _ = replay_phases(a, b, alg=1, tree=1, domain=2, seed=7)
+    np.testing.assert_allclose(np.abs(np.triu(S.dense(False))), np.abs(sl.qr(a, mode="r")[0]), atol=1e-11)

@@ -125,6 +125,12 @@ struct avs_handle {
     long long wallclock_khz = 100000;
     std::shared_ptr<DevProgram> prog;
     std::shared_ptr<DevPhases> phases;
+    // per-launch timing of the stream-ordered executor (opts.reserved[1] != 0):
+    // pev[i] .. pev[i+1] brackets launch i, whose kind is pkind[i]
+    std::vector<hipEvent_t> pev;
+    std::vector<int> pkind;
+    std::vector<double> pflops;
+    int plaunches = 0;
 };
 
 namespace {
@@ -235,11 +241,41 @@ template <typename TS, int B> avs_status run_host_enqueued(avs_handle *h)
     return AVS_OK;
 }
 
+// Algorithmic flops of an update phase: every op applies B reflectors to
+// ncols column tiles; a TS op (full B x B V) costs 4B^3 per tile (W = C1 + V^T C2,
+// C2 -= V W), a GE or TT op (triangular V) 2B^3.
+double phase_update_flops(const PhaseProgram &P, const Phase &ph)
+{
+    const double b3 = (double)P.B * P.B * P.B;
+    double f = 0.0;
+    for (int c = ph.chain0; c < ph.chain0 + ph.nchains; ++c)
+        for (int o = P.chain_start[c]; o < P.chain_start[c + 1]; ++o)
+            f += (P.ops[o].mode == MODE_TS ? 4.0 : 2.0) * b3 * P.ops[o].ncols;
+    return f;
+}
+
 template <typename TS, int B> avs_status run_streams(avs_handle *h)
 {
     const DevPhases &D = *h->phases;
     const int per = upd_tiles_for(B);
+    const bool timing = h->opts.reserved[1] != 0;
+    h->plaunches = 0;
+    if (timing) {
+        const size_t need = D.P.phases.size() + 1;
+        while (h->pev.size() < need) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            h->pev.push_back(e);
+        }
+        h->pkind.assign(D.P.phases.size(), 0);
+        h->pflops.assign(D.P.phases.size(), 0.0);
+        HIPCHK(hipEventRecord(h->pev[0], h->stream));
+    }
     for (const Phase &ph : D.P.phases) {
+        if (timing) {
+            h->pkind[h->plaunches] = ph.kind;
+            h->pflops[h->plaunches] = ph.kind == TK_UPDATE ? phase_update_flops(D.P, ph) : 0.0;
+        }
         if (ph.kind == TK_PANEL) {
             launch_chain_panel<TS, B>(ph.nchains, h->dTiles, h->dTau, h->dTjj, D.dOps, D.dChain, ph.chain0, h->stream);
         } else {
@@ -247,6 +283,7 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
             launch_chain_update<TS, B>(ph.nchains, groups, per, h->dTiles, h->dTjj, D.dOps, D.dChain, ph.chain0,
                                        split_of(h), h->stream);
         }
+        if (timing) HIPCHK(hipEventRecord(h->pev[++h->plaunches], h->stream));
     }
     HIPCHK(hipGetLastError());
     return AVS_OK;
@@ -388,6 +425,7 @@ avs_status avs_destroy(avs_handle *h)
     hipFree(h->dStatus);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
+    for (hipEvent_t e : h->pev) hipEventDestroy(e);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return AVS_OK;
@@ -486,6 +524,29 @@ avs_status avs_last_time_ms(avs_handle *h, double *ms)
     float f = 0.f;
     HIPCHK(hipEventElapsedTime(&f, h->ev0, h->ev1));
     *ms = f;
+    return check_status(h);
+}
+
+avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_launches, double *update_ms,
+                            int64_t *update_launches, double *update_flops)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.reserved[1] == 0 || h->plaunches == 0)
+        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.reserved[1] = 1)");
+    HIPCHK(hipEventSynchronize(h->pev[h->plaunches]));
+    double pm = 0, um = 0, uf = 0;
+    int64_t pn = 0, un = 0;
+    for (int i = 0; i < h->plaunches; ++i) {
+        float f = 0.f;
+        HIPCHK(hipEventElapsedTime(&f, h->pev[i], h->pev[i + 1]));
+        if (h->pkind[i] == TK_PANEL) { pm += f; ++pn; }
+        else { um += f; ++un; uf += h->pflops[i]; }
+    }
+    if (panel_ms) *panel_ms = pm;
+    if (panel_launches) *panel_launches = pn;
+    if (update_ms) *update_ms = um;
+    if (update_launches) *update_launches = un;
+    if (update_flops) *update_flops = uf;
     return check_status(h);
 }
 

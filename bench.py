@@ -90,6 +90,7 @@ def main():
     o.tree = avsvd.TREE_HIER if args.tree == "hier" else avsvd.TREE_FLAT
     o.domain = args.domain
     o.reserved[0] = 120000
+    o.reserved[1] = 1 if args.sched == "streams" else 0  # per-launch events for the roofline
     h = ctypes.c_void_p()
     check(L.avs_create(ctypes.byref(h), code, n, b, ctypes.byref(o)))
     pdA = ctypes.c_void_p(dA.data_ptr())
@@ -107,23 +108,46 @@ def main():
         log("warmup %d: kernel %.1f ms (first includes schedule build %.1f s)" %
             (w, ms.value, time.perf_counter() - t_build))
     kernel_ms = []
+    upd = {"ms": 0.0, "launches": 0, "flops": 0.0, "panel_ms": 0.0, "panel_launches": 0}
     check(L.avs_synchronize(h))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         step()
-        check(L.avs_last_time_ms(h, ctypes.byref(ms)))  # waits for this step's kernel
+        check(L.avs_last_time_ms(h, ctypes.byref(ms)))  # waits for this step's kernels
         kernel_ms.append(ms.value)
-        log("step %d: kernel %.1f ms" % (s, ms.value))
+        if args.sched == "streams":
+            pm, um, uf = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            pn, un = ctypes.c_int64(), ctypes.c_int64()
+            check(L.avs_phase_timing(h, ctypes.byref(pm), ctypes.byref(pn), ctypes.byref(um), ctypes.byref(un),
+                                     ctypes.byref(uf)))
+            upd["ms"] += um.value
+            upd["launches"] += un.value
+            upd["flops"] += uf.value
+            upd["panel_ms"] += pm.value
+            upd["panel_launches"] += pn.value
+            log("step %d: %.1f ms (update kernels %.1f ms in %d launches, panel kernels %.1f ms in %d)" %
+                (s, ms.value, um.value, un.value, pm.value, pn.value))
+        else:
+            log("step %d: %.1f ms" % (s, ms.value))
     check(L.avs_synchronize(h))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     flops = (8.0 / 3.0) * float(n) ** 3
     value = flops / (ms_per_step * 1e-3) / 1e12
-    kmean = sum(kernel_ms) / len(kernel_ms)
-    achieved = flops / (kmean * 1e-3) / 1e12
     peak = PEAK_FP64_TFLOPS if dt == np.float64 else PEAK_FP32_TFLOPS
+    if upd["launches"]:
+        # dominant kernel: chain_update_kernel; algorithmic flops per launch / mean launch duration
+        avg_ms = upd["ms"] / upd["launches"]
+        flops_per_launch = upd["flops"] / upd["launches"]
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        kname = "chain_update_kernel<%s,%d>" % ("double" if dt == np.float64 else "float", b)
+    else:
+        avg_ms = sum(kernel_ms) / len(kernel_ms)
+        flops_per_launch = flops
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        kname = "whole reduction (host-enqueued launches)"
 
     # size-independent check on the full-size result: ||band||_F == ||A||_F
     band = np.zeros((b + 1, n), order="F")
@@ -176,10 +200,12 @@ def main():
         "pct_of_peak": round(100.0 * value / peak, 2),
         "seconds_per_reduction": round(ms_per_step / 1e3, 4),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": traffic,
-                     "kernel": "chain phases (panel + update launches)" if args.sched == "streams"
-                     else "host-enqueued launches",
-                     "kernel_ms": [round(x, 3) for x in kernel_ms]},
+                     "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": kname,
+                     "avg_launch_ms": round(avg_ms, 5), "algorithmic_flops_per_launch": flops_per_launch,
+                     "launches": upd["launches"], "kernel_share_of_step": round(upd["ms"] / sum(kernel_ms), 4)
+                     if upd["launches"] else None,
+                     "panel_ms_per_step": round(upd["panel_ms"] / args.steps, 3),
+                     "step_kernel_ms": [round(x, 3) for x in kernel_ms]},
         "band_fro_rel_err": fro_err,
         "sigma_check": sigma_check,
         "cpu_baseline": cpu,

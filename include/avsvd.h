@@ -114,6 +114,14 @@ avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
 /* Copy the band to the host in FP64 LAPACK upper-band layout:
  * band[(b + i - j) + j*ldband] = A(i, j) for max(0, j-b) <= i <= j,
  * ldband >= b+1.  Only (b+1) x n values cross PCIe, not n x n. */
+/* Per-launch device time of the last avs_blockbidiag / avs_tiled_qr run under
+ * AVS_SCHED_STREAMS, for handles created with opts.reserved[1] != 0 (one event
+ * pair per launch): summed milliseconds and launch counts of the panel and
+ * update kernels, and the algorithmic flops of the update launches (4b^3 per
+ * TS-operation column tile, 2b^3 per GE/TT-operation column tile).  Waits for
+ * the run.  No counterpart in the reference (its benchmarks time whole calls). */
+avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_launches, double *update_ms,
+                            int64_t *update_launches, double *update_flops);
 avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband);
 
 /* Full svdvals: band reduction (if not already run), band to host, block

@@ -669,7 +669,7 @@ avs_status avs_tsmqr(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC
 
 avs_status avs_debug_mfma_layout(int32_t variant, double *out64, float *out32)
 {
-    if (variant < 0 || variant > 2) return fail(-1, "variant must be 0, 1 or 2");
+    if (variant < 0 || variant > 3) return fail(-1, "variant must be 0, 1, 2 or 3");
     if (!out64 || !out32) return fail(-2, "null output");
     double ha[64], hb[64];
     float fa[64], fb[64];
@@ -678,6 +678,8 @@ avs_status avs_debug_mfma_layout(int32_t variant, double *out64, float *out32)
         if (variant == 0) { ha[i] = (double)(i + 1); hb[i] = (double)((i * 7) % 13 - 6); }
         else if (variant == 1) { ha[i] = 1.0; hb[i] = (double)i; }     // sums of B lanes per output
         else { ha[i] = (double)i; hb[i] = 1.0; }                      // sums of A lanes per output
+        // variant 3: lane exchanges instead of MFMA (out64[4l+k] = lane l ^ (4<<k) of a,
+        // out32[4l+k] = sum of a over lanes congruent to l modulo 4<<k)
         fa[i] = (float)ha[i];
         fb[i] = (float)hb[i];
     }
@@ -689,7 +691,7 @@ avs_status avs_debug_mfma_layout(int32_t variant, double *out64, float *out32)
     HIPCHK(hipMemcpy(d + 64, hb, sizeof hb, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(f, fa, sizeof fa, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(f + 64, fb, sizeof fb, hipMemcpyHostToDevice));
-    launch_mfma_probe(d, d + 64, d + 128, f, f + 64, f + 128, 0);
+    launch_mfma_probe(d, d + 64, d + 128, f, f + 64, f + 128, variant == 3 ? 1 : 0, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(out64, d + 128, sizeof(double) * 256, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(out32, f + 128, sizeof(float) * 256, hipMemcpyDeviceToHost));

@@ -50,16 +50,36 @@ __device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c)
 constexpr int NTHREADS = 512; // 8 wave64 per workgroup
 constexpr int NWAVES = NTHREADS / 64;
 
+// Timing hook inside the panel loop (k = stage, j = reflector, pub = this lane
+// publishes the next column); empty unless a profiling build defines it.
+#ifndef AVS_PANEL_MARK
+#define AVS_PANEL_MARK(k, j, pub)
+#endif
+
 // element (r, c) of a tile with leading dimension ld, or of its adjoint
 __device__ __forceinline__ long toff(int r, int c, long ld, bool tr)
 {
     return tr ? (long)c + (long)r * ld : (long)r + (long)c * ld;
 }
 
+// Panel thread geometry for NT threads: the NT/64 waves split the B columns
+// evenly; the LPC lanes of one column (lanes l, l+CPW, l+2CPW, ... of its wave)
+// hold RPL consecutive rows each, so every column-wise reduction is a lane
+// butterfly inside a wave.
+template <int B, int NT = NTHREADS> struct PanelGeo {
+    static constexpr int CPW = B / (NT / 64); // columns per wave
+    static constexpr int LPC = 64 / CPW;      // lanes per column
+    static constexpr int RPL = B / LPC;       // rows per lane
+    static constexpr int XS = B + 2 * LPC;    // padded broadcast vector (2 doubles after each lane's rows)
+};
+// threads of the standalone / chained panel launches
+template <int B> struct PanelThreads { static constexpr int value = B >= 128 ? 1024 : NTHREADS; };
+
 // LDS footprint (in doubles) of each body; the executor allocates the max.
 template <typename TS, int B> struct Lds {
-    static constexpr int G = NTHREADS / B;
-    static constexpr int panel = B * (B + 1) / 2 + 2 * G + G * B + B + 4 + 16 * B + B + 16 * B;
+    // packed R, broadcast vectors and their norm partials (sized for the widest
+    // geometry), Gram rows, tau, beta
+    static constexpr int panel = B * (B + 1) / 2 + 2 * (B + 128) + 2 * 64 + 16 * B + 2 * B;
     static constexpr int update = (B * (B + 1) * (int)sizeof(TS) + 16 * B * (int)sizeof(TS) + 7) / 8;
     static constexpr int tjj = 16 * B + B + 16 * B;
     // split-BF16 update (FP32 storage): Y and Y^T as bf16 hi/lo pairs + T_JJ in FP32
@@ -69,177 +89,314 @@ template <typename TS, int B> struct Lds {
     static constexpr int maxv = panel > upd ? panel : upd; // bodies run by the persistent executor
 };
 
+// Lane exchanges v -> v[lane ^ m] on the VALU (no LDS round trip):
+//   ^4  : row_half_mirror (lane ^ 7 within 8) then quad_perm [3,2,1,0] (^ 3)
+//   ^8  : row_ror:8 inside each row of 16 lanes
+//   ^16 : v_permlane16_swap (odd rows of the first operand <-> even rows of the second)
+//   ^32 : v_permlane32_swap (upper half of the first operand <-> lower half of the second)
+template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+template <int M> __device__ __forceinline__ double lane_xor(double v)
+{
+    static_assert(M == 4 || M == 8 || M == 16 || M == 32, "lane_xor: 4, 8, 16 or 32");
+    if constexpr (M == 4) {
+        return dpp_mov<0x1B>(dpp_mov<0x141>(v));
+    } else if constexpr (M == 8) {
+        return dpp_mov<0x128>(v);
+    } else {
+        const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+        bool upper;
+        unsigned nlo, nhi;
+        if constexpr (M == 16) {
+            const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+            const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+            upper = (__lane_id() >> 4) & 1; // odd row: partner is the row below
+            nlo = upper ? rl[0] : rl[1];
+            nhi = upper ? rh[0] : rh[1];
+        } else {
+            const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+            const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+            upper = __lane_id() >= 32;
+            nlo = upper ? rl[0] : rl[1];
+            nhi = upper ? rh[0] : rh[1];
+        }
+        return __hiloint2double((int)nhi, (int)nlo);
+    }
+}
+
+// sum over the lanes of one panel column (lanes l, l ^ CPW, l ^ 2CPW, ...),
+// identical in every one of them: each butterfly step adds the same two
+// values in either order
+template <int CPW> __device__ __forceinline__ double colsum(double v)
+{
+    if constexpr (CPW <= 4) v += lane_xor<4>(v);
+    if constexpr (CPW <= 8) v += lane_xor<8>(v);
+    if constexpr (CPW <= 16) v += lane_xor<16>(v);
+    v += lane_xor<32>(v);
+    return v;
+}
+
+// sqrt(x) and 1/sqrt(x) for x > 0 finite: the hardware reciprocal square
+// root refined by two coupled Newton (Goldschmidt) steps; 1/x from the
+// hardware reciprocal refined by two Newton steps.  Accurate to a few ulp and
+// identical in every lane that runs them on the same input.
+__device__ __forceinline__ void sqrt_rsqrt(double x, double &sq, double &rs)
+{
+    const double r0 = __builtin_amdgcn_rsq(x);
+    double y = x * r0, h = 0.5 * r0;
+    double e = fma(-y, h, 0.5);
+    y = fma(y, e, y);
+    h = fma(h, e, h);
+    e = fma(-y, h, 0.5);
+    y = fma(y, e, y);
+    h = fma(h, e, h);
+    sq = y;
+    rs = 2.0 * h;
+}
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+// pairwise sum of N values (log2 N dependent adds)
+template <int N> __device__ __forceinline__ double tree_sum(double (&v)[N])
+{
+#pragma unroll
+    for (int h = 1; h < N; h <<= 1) {
+#pragma unroll
+        for (int u = 0; u + h < N; u += 2 * h) v[u] += v[u + h];
+    }
+    return v[0];
+}
+
+// a[s] for a wave-uniform s: a register-indexed move when the array is small
+// enough to stay in registers, a select chain otherwise
+template <int R> __device__ __forceinline__ double reg_get(const double (&a)[R], int s)
+{
+    if constexpr (R <= 16) {
+        return a[s];
+    } else {
+        double v = 0.0;
+#pragma unroll
+        for (int u = 0; u < R; ++u) v = (u == s) ? a[u] : v;
+        return v;
+    }
+}
+template <int R> __device__ __forceinline__ void reg_add(double (&a)[R], int s, double d)
+{
+    if constexpr (R <= 16) {
+        a[s] += d;
+    } else {
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+            if (u == s) a[u] += d;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Panel: Householder QR of one tile (GE), of [triu(top); bot] (TS) or of
-// [triu(top); triu(bot)] (TT).  Thread t owns column c = t % B and the RP
-// consecutive rows g*RP.. of it (g = t / B).  Per reflector j: (1) column-j
-// norm partials, (2) tau/beta/scale (FP64; an all-zero column gives tau = 0),
-// scaled reflector broadcast through LDS, (3) dot products v_j . a_c for all c
-// (c > j: the update coefficients; c < j in j's 16-block: Gram entries for
-// T_JJ), (4) rank-1 update.  Three workgroup barriers per reflector.
+// [triu(top); triu(bot)] (TT), computed in FP64 with one workgroup barrier
+// per reflector.
+// Reflector j: the lanes owning column j publish it in a double-buffered LDS
+// vector x, already restricted to the rows the reflector touches (GE: rows
+// >= j; TT: rows <= j by the triangle), with the partial sums of x^2 of each
+// lane group.  After the barrier every column group forms S_c = x . a_c over
+// its rows, sums it over its lanes, and derives
+//   beta = -sgn(alpha) ||(alpha, x)||,  tau = (beta - alpha) / beta,
+//   y = (x - beta e_j) / (alpha - beta)              (GE; y_j = 1)
+//   y = x / (alpha - beta)                           (TS, TT; 1 in R's row j)
+//   w_c = y . a_c = (S_c - beta a_c[j]) / (alpha - beta)   (GE)
+//   w_c = R[j][c] + S_c / (alpha - beta)                    (TS, TT)
+// then a_c -= tau w_c y (and R[j][c] -= tau w_c).  Columns c < j of j's
+// 16-block hold finished reflectors; the same formula gives the Gram entries
+// y_c . y_j of the compact-WY factor.  Waves whose columns all lie left of j's
+// 16-block only take part in the barrier.  An all-zero column gives tau = 0.
 // Outputs: reflectors in place (GE: strict lower part, unit diagonal
 // implicit; TS: the whole bottom tile; TT: the upper triangle of bot), R in
 // the upper triangle of top, tau[B] (FP64) and tjj[B*16] (FP64, row i holds
 // T[i][16*(i/16) + 0..15], the diagonal block of the compact-WY factor).
 // ---------------------------------------------------------------------------
-template <typename TS, int B, int MODE>
-__device__ void panel_body(TS *__restrict__ top, long ldt, TS *__restrict__ bot, long ldb, bool tr,
-                           double *__restrict__ tau, double *__restrict__ tjj, double *smem)
+template <typename TS, int B, int MODE, int NT = NTHREADS>
+__device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *__restrict__ bot, long ldb, bool tr,
+                                           double *__restrict__ tau, double *__restrict__ tjj, double *smem)
 {
-    constexpr int G = NTHREADS / B;
-    constexpr int RP = B / G;
-    const int t = threadIdx.x;
-    const int c = t % B, g = t / B;
+    typedef PanelGeo<B, NT> PG;
+    constexpr int CPW = PG::CPW, LPC = PG::LPC, RPL = PG::RPL, XS = PG::XS;
+    static_assert(XS <= B + 128 && LPC <= 64 && CPW >= 4, "panel geometry");
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int c = w * CPW + (l % CPW), q = l / CPW, r0 = q * RPL;
 
-    double *Rp = smem;                       // packed upper triangle of R (TS/TT)
-    double *partN = Rp + B * (B + 1) / 2;    // G
-    double *partD = partN + 2 * G;           // G*B
-    double *ybuf = partD + G * B;            // B
-    double *misc = ybuf + B;                 // 4
-    double *gramL = misc + 4;                // 16*B: gramL[j*16 + (c&15)] = v_c . v_j
-    double *tauL = gramL + 16 * B;           // B
-    double *TJ = tauL + B;                   // 16*B
-
+    double *Rp = smem;                    // packed upper triangle of R (TS/TT); diagonal read as alpha only
+    double *xbuf = Rp + B * (B + 1) / 2;  // 2 x XS: published column
+    double *nbuf = xbuf + 2 * XS;         // 2 x LPC: its partial sums of squares, per lane group
+    double *gramL = nbuf + 2 * LPC;       // gramL[j*16 + (c&15)] = y_c . y_j
+    double *tauL = gramL + 16 * B;        // B
+    double *dgL = tauL + B;               // B: R diagonal (beta_j) as produced
     auto rpk = [](int r, int cc) { return r * B - (r * (r - 1)) / 2 + (cc - r); };
+    auto xi = [](int row) { return row + (row / RPL) * 2; };
 
     TS *src = (MODE == MODE_GE) ? top : bot;
     const long lds = (MODE == MODE_GE) ? ldt : ldb;
-
-    // this thread's column c, rows g*RP + r at src_c + r*rs
     const long rs = tr ? lds : 1;
-    TS *src_c = src + toff(g * RP, c, lds, tr);
-    double a[RP];
+    TS *src_c = src + toff(r0, c, lds, tr);
+    double a[RPL];
 #pragma unroll
-    for (int r = 0; r < RP; ++r) {
-        const int row = g * RP + r;
-        a[r] = (MODE == MODE_TT && row > c) ? 0.0 : (double)src_c[(long)r * rs];
+    for (int s = 0; s < RPL; ++s) {
+        const int row = r0 + s;
+        a[s] = (MODE == MODE_TT && row > c) ? 0.0 : (double)src_c[(long)s * rs];
     }
     if (MODE != MODE_GE) {
-        for (int idx = t; idx < B * B; idx += NTHREADS) {
+        for (int idx = t; idx < B * B; idx += NT) {
             int r, cc;
             if (tr) { cc = idx % B; r = idx / B; } else { r = idx % B; cc = idx / B; }
-            if (r <= cc) Rp[rpk(r, cc)] = (double)top[toff(r, cc, ldt, tr)];
+            if (r <= cc) {
+                const double v = (double)top[toff(r, cc, ldt, tr)];
+                Rp[rpk(r, cc)] = v;
+                if (r == cc) dgL[r] = v;
+            }
         }
     }
-    for (int idx = t; idx < 16 * B; idx += NTHREADS) gramL[idx] = 0.0;
-    for (int idx = t; idx < B; idx += NTHREADS) tauL[idx] = 0.0;
-    __syncthreads();
+    for (int idx = t; idx < 16 * B; idx += NT) gramL[idx] = 0.0;
+    for (int idx = t; idx < B; idx += NT) tauL[idx] = 0.0;
+
+    // lanes of column p publish it for reflector p
+    auto publish = [&](int p) {
+        double *xn = xbuf + (p & 1) * XS;
+        double sq[RPL];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+            const int row = r0 + s;
+            const double v = (MODE == MODE_GE && row < p) ? 0.0 : a[s];
+            xn[xi(row)] = v;
+            sq[s] = v * v;
+        }
+        nbuf[(p & 1) * LPC + q] = tree_sum(sq);
+    };
+    if (c == 0) publish(0);
 
     const int jend = (MODE == MODE_GE) ? B - 1 : B;
     for (int j = 0; j < jend; ++j) {
-        // (1) norm partials of column j
-        if (c == j) {
-            double ss = 0.0;
-#pragma unroll
-            for (int r = 0; r < RP; ++r) {
-                const int row = g * RP + r;
-                if (MODE != MODE_GE || row > j) ss += a[r] * a[r];
-                if (MODE == MODE_GE && row == j) misc[0] = a[r];
-            }
-            partN[g] = ss;
+        __syncthreads(); // column j published, previous reflector's writes visible
+        const int j16 = j & ~15;
+        if ((w + 1) * CPW <= j16) continue; // every column of this wave is finished
+        AVS_PANEL_MARK(0, j, c == j + 1 && q == 0);
+        const double *xb = xbuf + (j & 1) * XS;
+        const double *nb = nbuf + (j & 1) * LPC;
+        // everything this reflector reads from LDS is requested up front
+        const double alpha = (MODE == MODE_GE) ? xb[xi(j)] : Rp[rpk(j, j)];
+        double rjc = 0.0; // R[j][c]; the packed index stays inside R for every c < B
+        if (MODE != MODE_GE) {
+            const double v = Rp[rpk(j, c)];
+            rjc = c > j ? v : 0.0;
         }
-        __syncthreads();
-        // (2) reflector
-        double s = 0.0;
+        double nv[LPC];
 #pragma unroll
-        for (int gg = 0; gg < G; ++gg) s += partN[gg];
-        const double alpha = (MODE != MODE_GE) ? Rp[rpk(j, j)] : misc[0];
-        const double nrm = sqrt(s + alpha * alpha);
-        if (nrm == 0.0) { // H_j = I
-            if (t == 0) tau[j] = 0.0;
-            __syncthreads();
-            continue;
+        for (int u = 0; u < LPC; ++u) nv[u] = nb[u];
+        double xr[RPL], pr[RPL];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+            xr[s] = xb[xi(r0 + s)];
+            pr[s] = xr[s] * a[s];
         }
-        const double beta = alpha >= 0.0 ? -nrm : nrm;
-        const double taur = (beta - alpha) / beta;
-        const double scal = 1.0 / (alpha - beta);
-        if (c == j) {
+        // Householder scalars from ||x||^2 and alpha alone, overlapping the
+        // reduction of S.  With rn = 1/nrm:  beta = -sgn(alpha) nrm,
+        // tau = (beta - alpha)/beta = 1 + |alpha| rn,
+        // scal = 1/(alpha - beta) = sgn(alpha) rn / tau,  tau scal^2 = rn^2 / tau.
+        const double nn = tree_sum(nv);
+        const double n2 = (MODE == MODE_GE) ? nn : fma(alpha, alpha, nn);
+        AVS_PANEL_MARK(1, j, c == j + 1 && q == 0 && n2 == n2);
+        double nrm = 0.0, rn = 0.0;
+        if (n2 != 0.0) sqrt_rsqrt(n2, nrm, rn);
+        const double sg = alpha >= 0.0 ? 1.0 : -1.0;
+        const double beta = -sg * nrm;
+        const double taur = fma(fabs(alpha), rn, 1.0);
+        const double itau = rcp_nr(taur);
+        const double srn = sg * rn;
+        const double scal = srn * itau;
+        const double c2 = rn * rn * itau;
+        // S_c = x . a_c and (GE) a_c[j]
+        const double S = colsum<CPW>(tree_sum(pr));
+        AVS_PANEL_MARK(2, j, c == j + 1 && q == 0 && S == S && c2 == c2);
+        const int qj = j / RPL, sj = j % RPL; // lane group and slot of row j
+        double pj = 0.0;
+        if (MODE == MODE_GE) pj = colsum<CPW>(q == qj ? reg_get(a, sj) : 0.0);
+        if (n2 != 0.0) {
+            // GE: w_c = scal (S - beta a_c[j]);  TS/TT: w_c = R[j][c] + scal S
+            // a_c -= tau w_c y  ==  a_c -= (tau w_c scal) x  (+ GE pivot fix-up)
+            const double sgp = (MODE == MODE_GE) ? fma(-beta, pj, S) : S;
+            if (c > j) {
+                const double tws = (MODE == MODE_GE) ? c2 * sgp : fma(c2, S, srn * rjc);
 #pragma unroll
-            for (int r = 0; r < RP; ++r) {
-                const int row = g * RP + r;
-                double y;
-                if (MODE != MODE_GE) {
-                    y = a[r] * scal;
-                    a[r] = y;
-                } else if (row > j) {
-                    y = a[r] * scal;
-                    a[r] = y;
-                } else if (row == j) {
-                    y = 1.0;
-                    a[r] = beta;
-                } else {
-                    y = 0.0;
+                for (int s = 0; s < RPL; ++s) a[s] = fma(-tws, xr[s], a[s]);
+                // GE: y_j = 1, not alpha * scal
+                if (MODE == MODE_GE && q == qj) reg_add(a, sj, tws * beta);
+                if (MODE != MODE_GE && q == 0) Rp[rpk(j, c)] = rjc - fma(taur, rjc, srn * S);
+            } else if (c == j) {
+#pragma unroll
+                for (int s = 0; s < RPL; ++s) {
+                    const int row = r0 + s;
+                    a[s] = (MODE == MODE_GE) ? (row > j ? xr[s] * scal : (row == j ? beta : a[s])) : xr[s] * scal;
                 }
-                ybuf[row] = y;
+                if (q == 0) {
+                    tauL[j] = taur;
+                    if (MODE != MODE_GE) dgL[j] = beta;
+                }
+            } else if (c >= j16 && q == 0) {
+                gramL[j * 16 + (c & 15)] = scal * sgp;
             }
         }
-        if (t == 0) { tau[j] = taur; tauL[j] = taur; }
-        __syncthreads();
-        // (3) dot products
-        double rjc = 0.0;
-        if (c != j) {
-            double d = 0.0;
-#pragma unroll
-            for (int r = 0; r < RP; ++r) d += ybuf[g * RP + r] * a[r];
-            partD[g * B + c] = d;
-            if (MODE != MODE_GE && c > j) rjc = Rp[rpk(j, c)];
-        }
-        __syncthreads();
-        // (4) rank-1 update / Gram entries
-        if (c > j) {
-            double w = rjc;
-#pragma unroll
-            for (int gg = 0; gg < G; ++gg) w += partD[gg * B + c];
-            const double tw = taur * w;
-#pragma unroll
-            for (int r = 0; r < RP; ++r) a[r] -= tw * ybuf[g * RP + r];
-            if (MODE != MODE_GE && g == 0) Rp[rpk(j, c)] = rjc - tw;
-        } else if (c < j && g == 0 && c >= (j & ~15)) {
-            double d = 0.0;
-#pragma unroll
-            for (int gg = 0; gg < G; ++gg) d += partD[gg * B + c];
-            gramL[j * 16 + (c & 15)] = d;
-        }
-        if (MODE != MODE_GE && c == j && g == 0) Rp[rpk(j, j)] = beta;
+        AVS_PANEL_MARK(3, j, c == j + 1 && q == 0 && a[0] == a[0]);
+        if (c == j + 1) publish(j + 1);
+        AVS_PANEL_MARK(4, j, c == j + 1 && q == 0);
     }
-    if (MODE == MODE_GE && t == 0) tau[B - 1] = 0.0;
     __syncthreads();
 
-    // write back
+    // write back (tau from LDS: tauL holds 0 for every identity reflector and
+    // for GE's last column).  The tile pointer goes through an empty asm so
+    // the per-row addresses are formed again here instead of being kept live
+    // in registers across the reflector loop.
+    for (int idx = t; idx < B; idx += NT) tau[idx] = tauL[idx];
+    TS *src_w = src;
+    asm volatile("" : "+s"(src_w));
+    TS *dst_c = src_w + toff(r0, c, lds, tr);
 #pragma unroll
-    for (int r = 0; r < RP; ++r) {
-        const int row = g * RP + r;
-        if (MODE != MODE_TT || row <= c) src_c[(long)r * rs] = (TS)a[r];
+    for (int s = 0; s < RPL; ++s) {
+        const int row = r0 + s;
+        if (MODE != MODE_TT || row <= c) dst_c[(long)s * rs] = (TS)a[s];
     }
     if (MODE != MODE_GE) {
-        for (int idx = t; idx < B * B; idx += NTHREADS) {
+        for (int idx = t; idx < B * B; idx += NT) {
             int r, cc;
             if (tr) { cc = idx % B; r = idx / B; } else { r = idx % B; cc = idx / B; }
-            if (r <= cc) top[toff(r, cc, ldt, tr)] = (TS)Rp[rpk(r, cc)];
+            if (r < cc) top[toff(r, cc, ldt, tr)] = (TS)Rp[rpk(r, cc)];
+            else if (r == cc) top[toff(r, cc, ldt, tr)] = (TS)dgL[r];
         }
     }
 
-    // diagonal blocks of the compact-WY factor: T[i][j] = -tau_j sum_{m=i}^{j-1} T[i][m] G[m][j]
-    if (tjj) {
+    // diagonal blocks of the compact-WY factor, one row per lane:
+    // T[i][j] = -tau_j sum_{m=i}^{j-1} T[i][m] G[m][j]  (i < j in one 16-block).
+    // trow[m] = T[i][m] is exactly zero for m < i, so the sum runs over m < j.
+    if (tjj && t < B) {
+        const int J = t >> 4, il = t & 15;
+        double trow[16];
+#pragma unroll
         for (int jl = 0; jl < 16; ++jl) {
-            if (t < B) {
-                const int J = t >> 4, il = t & 15;
-                const int j = J * 16 + jl;
-                double v;
-                if (il == jl) {
-                    v = tauL[j];
-                } else if (il < jl) {
-                    double acc = 0.0;
-                    for (int m = il; m < jl; ++m) acc += TJ[(J * 16 + il) * 16 + m] * gramL[j * 16 + m];
-                    v = -tauL[j] * acc;
-                } else {
-                    v = 0.0;
-                }
-                TJ[(J * 16 + il) * 16 + jl] = v;
-            }
-            __syncthreads();
+            const int j = J * 16 + jl;
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < jl; ++m) acc += trow[m] * gramL[j * 16 + m];
+            const double tj = tauL[j];
+            const double v = il == jl ? tj : (il < jl ? -tj * acc : 0.0);
+            trow[jl] = v;
         }
-        for (int idx = t; idx < 16 * B; idx += NTHREADS) tjj[idx] = TJ[idx];
+#pragma unroll
+        for (int jl = 0; jl < 16; ++jl) tjj[t * 16 + jl] = trow[jl];
     }
 }
 
@@ -248,7 +405,7 @@ __device__ void panel_body(TS *__restrict__ top, long ldt, TS *__restrict__ bot,
 // callers pass V and tau only, like the reference kernels).
 // ---------------------------------------------------------------------------
 template <typename TS, int B, int MODE>
-__device__ void tjj_body(const TS *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tau,
+__device__ __forceinline__ void tjj_body(const TS *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tau,
                          double *__restrict__ tjj, double *smem)
 {
     const int t = threadIdx.x;
@@ -322,7 +479,7 @@ template <> struct AccRow<float> { static constexpr int base = 4, step = 1; };
 template <> struct AccRow<double> { static constexpr int base = 1, step = 4; };
 
 template <typename TS, int B, int MODE, bool APPLYT>
-__device__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tjj,
+__device__ __forceinline__ void update_body(const TS *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tjj,
                             TS *__restrict__ C1, long ld1, long s1, TS *__restrict__ C2, long ld2, long s2,
                             bool ctr, int ntiles, double *smem)
 {
@@ -452,7 +609,7 @@ __device__ __forceinline__ void split_bf(float x, unsigned short &h, unsigned sh
 }
 
 template <int B, int MODE, bool APPLYT>
-__device__ void update_body_split(const float *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tjj,
+__device__ __forceinline__ void update_body_split(const float *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tjj,
                                   float *__restrict__ C1, long ld1, long s1, float *__restrict__ C2, long ld2, long s2,
                                   bool ctr, int ntiles, double *smem)
 {

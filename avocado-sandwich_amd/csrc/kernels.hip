@@ -64,11 +64,11 @@ __global__ void __launch_bounds__(256) band_kernel(const TS *__restrict__ tiles,
 
 // ----------------------------------------------------- standalone operators
 template <typename TS, int B, int MODE>
-__global__ void __launch_bounds__(NTHREADS) panel_kernel(TS *top, long ldt, TS *bot, long ldb, int tr,
-                                                         double *tau, double *tjj)
+__global__ void __launch_bounds__(PanelThreads<B>::value) panel_kernel(TS *top, long ldt, TS *bot, long ldb, int tr,
+                                                                       double *tau, double *tjj)
 {
     __shared__ double smem[Lds<TS, B>::panel];
-    panel_body<TS, B, MODE>(top, ldt, bot, ldb, tr != 0, tau, tjj, smem);
+    panel_body<TS, B, MODE, PanelThreads<B>::value>(top, ldt, bot, ldb, tr != 0, tau, tjj, smem);
 }
 
 template <typename TS, int B, int MODE>
@@ -193,10 +193,12 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
 // Panel phase: workgroup c runs the panel operations of chain c in order
 // (a domain's GEQRT + TSQRT chain, or one tree merge).
 template <typename TS, int B>
-__global__ void __launch_bounds__(NTHREADS) chain_panel_kernel(TS *tiles, double *taus, double *tjjs,
-                                                               const ChainOp *__restrict__ ops,
-                                                               const int *__restrict__ chain_start, int chain0)
+__global__ void __launch_bounds__(PanelThreads<B>::value) chain_panel_kernel(TS *tiles, double *taus, double *tjjs,
+                                                                             const ChainOp *__restrict__ ops,
+                                                                             const int *__restrict__ chain_start,
+                                                                             int chain0)
 {
+    constexpr int NT = PanelThreads<B>::value;
     __shared__ double smem[Lds<TS, B>::panel];
     const long tsz = (long)B * B;
     const int c = chain0 + blockIdx.x;
@@ -207,9 +209,9 @@ __global__ void __launch_bounds__(NTHREADS) chain_panel_kernel(TS *tiles, double
         double *tau = taus + (long)T.fac * B;
         double *tjj = tjjs + (long)T.fac * 16 * B;
         const bool tr = T.trans != 0;
-        if (T.mode == MODE_GE) panel_body<TS, B, MODE_GE>(top, B, top, B, tr, tau, tjj, smem);
-        else if (T.mode == MODE_TS) panel_body<TS, B, MODE_TS>(top, B, bot, B, tr, tau, tjj, smem);
-        else panel_body<TS, B, MODE_TT>(top, B, bot, B, tr, tau, tjj, smem);
+        if (T.mode == MODE_GE) panel_body<TS, B, MODE_GE, NT>(top, B, top, B, tr, tau, tjj, smem);
+        else if (T.mode == MODE_TS) panel_body<TS, B, MODE_TS, NT>(top, B, bot, B, tr, tau, tjj, smem);
+        else panel_body<TS, B, MODE_TT, NT>(top, B, bot, B, tr, tau, tjj, smem);
         __syncthreads();
     }
 }
@@ -252,7 +254,7 @@ template <typename TS, int B>
 void launch_chain_panel(int nchains, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
                         const int *chain_start, int chain0, hipStream_t s)
 {
-    hipLaunchKernelGGL((chain_panel_kernel<TS, B>), dim3(nchains), dim3(NTHREADS), 0, s, (TS *)tiles, taus, tjjs, ops,
+    hipLaunchKernelGGL((chain_panel_kernel<TS, B>), dim3(nchains), dim3(PanelThreads<B>::value), 0, s, (TS *)tiles, taus, tjjs, ops,
                        chain_start, chain0);
 }
 template <typename TS, int B>
@@ -412,9 +414,22 @@ void launch_batch_band(const void *arena, const long *off, const int *nts, const
 // One 16x16x4 MFMA per element type; lane l supplies a[l] and b[l] and the
 // accumulator of every lane is written out so the host can read the lane layout.
 __global__ void mfma_probe_kernel(const double *a, const double *b, double *out64, const float *af,
-                                  const float *bf, float *out32)
+                                  const float *bf, float *out32, int lanes)
 {
     const int l = threadIdx.x;
+    if (lanes) { // lane-exchange check: v = a[l]; out64[4l + k] = v[l ^ (4 << k)]
+        const double v = a[l];
+        out64[l * 4 + 0] = lane_xor<4>(v);
+        out64[l * 4 + 1] = lane_xor<8>(v);
+        out64[l * 4 + 2] = lane_xor<16>(v);
+        out64[l * 4 + 3] = lane_xor<32>(v);
+        // out32[4l + k]: sum of v over the lanes congruent to l modulo 4 << k
+        out32[l * 4 + 0] = (float)colsum<4>(v);
+        out32[l * 4 + 1] = (float)colsum<8>(v);
+        out32[l * 4 + 2] = (float)colsum<16>(v);
+        out32[l * 4 + 3] = (float)colsum<32>(v);
+        return;
+    }
     d4_t acc = vzero4<d4_t>();
     acc = mfma16(a[l], b[l], acc);
     for (int m = 0; m < 4; ++m) out64[l * 4 + m] = acc[m];
@@ -444,13 +459,13 @@ template <typename TS, int B>
 void launch_panel(int mode, void *top, long ldt, void *bot, long ldb, int tr, double *tau, double *tjj, hipStream_t s)
 {
     if (mode == MODE_GE)
-        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_GE>), dim3(1), dim3(NTHREADS), 0, s, (TS *)top, ldt, (TS *)top,
+        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_GE>), dim3(1), dim3(PanelThreads<B>::value), 0, s, (TS *)top, ldt, (TS *)top,
                            ldt, tr, tau, tjj);
     else if (mode == MODE_TS)
-        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_TS>), dim3(1), dim3(NTHREADS), 0, s, (TS *)top, ldt, (TS *)bot,
+        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_TS>), dim3(1), dim3(PanelThreads<B>::value), 0, s, (TS *)top, ldt, (TS *)bot,
                            ldb, tr, tau, tjj);
     else
-        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_TT>), dim3(1), dim3(NTHREADS), 0, s, (TS *)top, ldt, (TS *)bot,
+        hipLaunchKernelGGL((panel_kernel<TS, B, MODE_TT>), dim3(1), dim3(PanelThreads<B>::value), 0, s, (TS *)top, ldt, (TS *)bot,
                            ldb, tr, tau, tjj);
 }
 
@@ -516,9 +531,9 @@ template <typename TS, int B> int persistent_occupancy()
 }
 
 void launch_mfma_probe(const double *a, const double *b, double *o64, const float *af, const float *bf, float *o32,
-                       hipStream_t s)
+                       int lanes, hipStream_t s)
 {
-    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, a, b, o64, af, bf, o32);
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, a, b, o64, af, bf, o32, lanes);
 }
 
 #define AVS_INST_B(TS, B)                                                                                             \

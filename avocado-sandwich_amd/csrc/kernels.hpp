@@ -47,6 +47,6 @@ void launch_chain_update(int nchains, int groups, int per, void *tiles, const do
                          const int *chain_start, int chain0, int split, hipStream_t s);
 
 void launch_mfma_probe(const double *a, const double *b, double *o64, const float *af, const float *bf, float *o32,
-                       hipStream_t s);
+                       int lanes, hipStream_t s);
 
 } // namespace avs

@@ -235,7 +235,10 @@ avs_status avs_server_destroy(avs_server *server);
  * as its B operand (variant 0: a_l = l+1, b_l = (7l mod 13) - 6; variant 1:
  * a_l = 1, b_l = l; variant 2: a_l = l, b_l = 1) and returns each lane's 4
  * accumulator values (out[lane*4 + m]), so a test can pin the operand and
- * accumulator lane layouts the update kernels rely on.
+ * accumulator lane layouts the update kernels rely on.  Variant 3 checks the
+ * panel kernels' lane exchanges instead: out64[4l + k] = a[l ^ (4 << k)] and
+ * out32[4l + k] = sum of a over the lanes congruent to l modulo 4 << k, with
+ * a_l = l.
  * avs_schedule_stats: size of the task DAG and the list-scheduling makespan
  * estimate (microseconds) for an nt x nt tile grid, without a device. */
 avs_status avs_debug_mfma_layout(int32_t variant, double *out64, float *out32);

@@ -68,6 +68,18 @@ def test_mfma_lane_layout(gpu):
                                   np.array([[4 * (4 * (l // 16) + m) + 96 for m in range(4)] for l in range(64)]))
 
 
+def test_lane_exchanges(gpu):
+    """The panel reductions' VALU lane exchanges (DPP row_half_mirror +
+    quad_perm, row_ror:8, v_permlane16_swap, v_permlane32_swap) move lane
+    l ^ m to lane l, and the column sums are exact in every lane."""
+    o64, o32 = avsvd.mfma_layout(3)
+    lanes = np.arange(64)
+    for k, m in enumerate((4, 8, 16, 32)):
+        np.testing.assert_array_equal(o64[:, k], (lanes ^ m).astype(np.float64), err_msg="xor %d" % m)
+        want = np.array([lanes[lanes % (4 << k) == l % (4 << k)].sum() for l in lanes], dtype=np.float64)
+        np.testing.assert_array_equal(o32[:, k].astype(np.float64), want, err_msg="sum mod %d" % (4 << k))
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("b", [32, 64])
 def test_geqrt_vs_lapack(gpu, golden_dir, dt, b):

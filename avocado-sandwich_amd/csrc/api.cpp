@@ -116,6 +116,7 @@ struct avs_handle {
     void *dTiles = nullptr;
     void *dStage = nullptr; // column-major staging (n*n elements)
     double *dTau = nullptr, *dTjj = nullptr, *dBand = nullptr;
+    double *hBand = nullptr; // pinned host staging for the band (b+1) x n
     int *dCnt = nullptr, *dTicket = nullptr, *dStatus = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool have_matrix = false;
@@ -139,8 +140,10 @@ int resolve_tree(const avs_handle *h) { return h->opts.tree; }
 int resolve_domain(const avs_handle *h)
 {
     if (h->opts.domain > 0) return h->opts.domain;
-    // domain of ~16 tiles (tiles fewer than that -> flat)
-    return 16;
+    // Two tiles per domain: the panel chain of a sweep is then 2 + log2(m/2)
+    // panels long (the shortest for m >= 4 up to one), and the update work is
+    // the same for every domain size.
+    return 2;
 }
 int upd_tiles_for(int b) { return b >= 128 ? 1 : (b == 64 ? 2 : 4); }
 
@@ -420,6 +423,7 @@ avs_status avs_destroy(avs_handle *h)
     hipFree(h->dTau);
     hipFree(h->dTjj);
     hipFree(h->dBand);
+    if (h->hBand) hipHostFree(h->hBand);
     hipFree(h->dCnt);
     hipFree(h->dTicket);
     hipFree(h->dStatus);
@@ -560,9 +564,12 @@ avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband)
     if (h->dtype == AVS_F64) launch_band<double>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
     else launch_band<float>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpy2DAsync(band, sizeof(double) * ldband, h->dBand, sizeof(double) * (h->b + 1),
-                            sizeof(double) * (h->b + 1), h->n, hipMemcpyDeviceToHost, h->stream));
+    // band-only D2H through a pinned staging buffer, then into the caller's layout
+    const size_t ld = (size_t)h->b + 1;
+    if (!h->hBand) HIPCHK(hipHostMalloc((void **)&h->hBand, sizeof(double) * ld * h->n, hipHostMallocDefault));
+    HIPCHK(hipMemcpyAsync(h->hBand, h->dBand, sizeof(double) * ld * h->n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    for (int64_t j = 0; j < h->n; ++j) memcpy(band + j * ldband, h->hBand + j * ld, sizeof(double) * ld);
     return check_status(h);
 }
 
@@ -783,7 +790,8 @@ struct avs_batch {
     int workers = 0, split = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool reduced = false, timed = false;
-    std::vector<double> hBands;
+    double *hBands = nullptr; // pinned host copy of every band (band_elems doubles)
+    bool bands_valid = false;
 };
 
 extern "C" {
@@ -880,6 +888,7 @@ avs_status avs_batch_destroy(avs_batch *B)
     for (void *p : ptrs) hipFree(p);
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
+    if (B->hBands) hipHostFree(B->hBands);
     if (B->own_stream) hipStreamDestroy(B->stream);
     delete B;
     return AVS_OK;
@@ -897,6 +906,7 @@ avs_status avs_batch_set_matrix_host(avs_batch *B, int32_t idx, const void *A, i
     char *dst = (char *)B->dStage + es * (size_t)B->soff[idx];
     HIPCHK(hipMemcpy2DAsync(dst, es * N, A, es * lda, es * n, n, hipMemcpyHostToDevice, B->stream));
     B->reduced = false;
+    B->bands_valid = false;
     return AVS_OK;
 }
 
@@ -918,6 +928,7 @@ avs_status avs_batch_blockbidiag(avs_batch *B)
     });
     if (st) return st;
     B->reduced = true;
+    B->bands_valid = false;
     B->timed = true;
     return AVS_OK;
 }
@@ -944,10 +955,11 @@ static avs_status batch_fetch_bands(avs_batch *B)
         return AVS_OK;
     });
     if (st) return st;
-    B->hBands.resize((size_t)B->band_elems);
-    HIPCHK(hipMemcpyAsync(B->hBands.data(), B->dBands, sizeof(double) * B->band_elems, hipMemcpyDeviceToHost,
-                          B->stream));
+    if (!B->hBands)
+        HIPCHK(hipHostMalloc((void **)&B->hBands, sizeof(double) * (size_t)B->band_elems, hipHostMallocDefault));
+    HIPCHK(hipMemcpyAsync(B->hBands, B->dBands, sizeof(double) * B->band_elems, hipMemcpyDeviceToHost, B->stream));
     HIPCHK(hipStreamSynchronize(B->stream));
+    B->bands_valid = true;
     return AVS_OK;
 }
 
@@ -958,12 +970,12 @@ avs_status avs_batch_get_band(avs_batch *B, int32_t idx, double *band, int64_t l
     if (!band) return fail(-3, "null band");
     if (ldband < B->b + 1) return fail(-4, "ldband < b+1");
     if (!B->reduced) return fail(AVS_ERR_STATE, "batch not reduced yet");
-    if (B->hBands.empty()) {
+    if (!B->bands_valid) {
         avs_status st = batch_fetch_bands(B);
         if (st) return st;
     }
     const long N = (long)B->nt[idx] * B->b;
-    const double *src = B->hBands.data() + B->boff[idx];
+    const double *src = B->hBands + B->boff[idx];
     for (long j = 0; j < N; ++j) memcpy(band + j * ldband, src + j * (B->b + 1), sizeof(double) * (B->b + 1));
     return AVS_OK;
 }
@@ -976,7 +988,7 @@ avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
     if (!B->reduced) {
         if ((st = avs_batch_blockbidiag(B))) return st;
     }
-    B->hBands.clear();
+    B->bands_valid = false;
     if ((st = batch_fetch_bands(B))) return st;
     std::vector<long> soff(B->count, 0);
     for (int i = 1; i < B->count; ++i) soff[i] = soff[i - 1] + B->n[i - 1];
@@ -997,7 +1009,7 @@ avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
             d.resize(N);
             e.resize(std::max<long>(N - 1, 1));
             sg.resize(N);
-            avs_status s2 = bidiagonalize_band_threads(N, B->b, B->hBands.data() + B->boff[m], B->b + 1, d.data(),
+            avs_status s2 = bidiagonalize_band_threads(N, B->b, B->hBands + B->boff[m], B->b + 1, d.data(),
                                                        e.data(), 1);
             if (!s2) s2 = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
             if (s2) {

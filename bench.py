@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check-n", type=int, default=4096,
                     help="size of the sigma check against the cached FP64 dgesdd vector (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (a known-traffic kernel for scaling "
+                         "hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only (this engine runs one matrix per GPU; see the sharding design "
@@ -168,6 +171,16 @@ def main():
         sigma_check = {"n": args.check_n, "max_abs_err_over_sigma_max": err,
                        "vs": "FP64 LAPACK dgesdd of the same seeded matrix", "svdvals_seconds": t_s}
         log("sigma check n=%d: %.3e" % (args.check_n, err))
+
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src = torch.empty(nb // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: %d MiB device-to-device copy done" % args.copy_calibration)
+        del src, dst
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_blockbidiag_n%d_%s_b%d.json" % (n, args.dtype, b))

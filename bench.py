@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--sched", choices=["streams", "host"], default="streams")
     ap.add_argument("--tree", choices=["hier", "flat"], default="hier")
     ap.add_argument("--domain", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--cpu-n", type=int, default=1536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check-n", type=int, default=4096,
                     help="size of the sigma check against the cached FP64 dgesdd vector (0 = skip)")

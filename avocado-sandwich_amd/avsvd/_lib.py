@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libavsvd.so")
+# the in-tree build, unless AVSVD_LIBRARY names another build of the same library
+LIB_PATH = os.environ.get("AVSVD_LIBRARY") or os.path.join(PKG_DIR, "libavsvd.so")
 
 AVS_F64, AVS_F32 = 0, 1
 SCHED_PERSISTENT, SCHED_HOST, SCHED_STREAMS = 0, 1, 2

@@ -280,11 +280,12 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
             h->pflops[h->plaunches] = ph.kind == TK_UPDATE ? phase_update_flops(D.P, ph) : 0.0;
         }
         if (ph.kind == TK_PANEL) {
-            launch_chain_panel<TS, B>(ph.nchains, h->dTiles, h->dTau, h->dTjj, D.dOps, D.dChain, ph.chain0, h->stream);
+            launch_chain_panel<TS, B>(ph.nchains, ph.mode, h->dTiles, h->dTau, h->dTjj, D.dOps, D.dChain, ph.chain0,
+                                      h->stream);
         } else {
             const int groups = (ph.ncols + per - 1) / per;
-            launch_chain_update<TS, B>(ph.nchains, groups, per, h->dTiles, h->dTjj, D.dOps, D.dChain, ph.chain0,
-                                       split_of(h), h->stream);
+            launch_chain_update<TS, B>(ph.nchains, groups, per, ph.mode, ph.trans, h->dTiles, h->dTjj, D.dOps,
+                                       D.dChain, ph.chain0, split_of(h), h->stream);
         }
         if (timing) HIPCHK(hipEventRecord(h->pev[++h->plaunches], h->stream));
     }

@@ -608,7 +608,7 @@ __device__ __forceinline__ void split_bf(float x, unsigned short &h, unsigned sh
     l = f2bf(x - bf2f(h));
 }
 
-template <int B, int MODE, bool APPLYT>
+template <int B, int MODE, bool APPLYT, int NT = NTHREADS>
 __device__ __forceinline__ void update_body_split(const float *__restrict__ V, long ldv, bool vtr, const double *__restrict__ tjj,
                                   float *__restrict__ C1, long ld1, long s1, float *__restrict__ C2, long ld2, long s2,
                                   bool ctr, int ntiles, double *smem)
@@ -622,7 +622,7 @@ __device__ __forceinline__ void update_body_split(const float *__restrict__ V, l
     unsigned short *Yl = Yh + B * S;
     float *TL = reinterpret_cast<float *>(Yl + B * S);
     const int t = threadIdx.x;
-    for (int idx = t; idx < B * B; idx += NTHREADS) {
+    for (int idx = t; idx < B * B; idx += NT) {
         int k, i;
         if (vtr) { i = idx % B; k = idx / B; } else { k = idx % B; i = idx / B; }
         float v = V[toff(k, i, ldv, vtr)];
@@ -635,14 +635,14 @@ __device__ __forceinline__ void update_body_split(const float *__restrict__ V, l
         Yh[k * S + i] = h;
         Yl[k * S + i] = l;
     }
-    for (int idx = t; idx < 16 * B; idx += NTHREADS) TL[idx] = (float)tjj[idx];
+    for (int idx = t; idx < 16 * B; idx += NT) TL[idx] = (float)tjj[idx];
     __syncthreads();
 
     const int lane = t & 63, w = t >> 6, q = lane >> 4, il = lane & 15;
     const long rs1 = ctr ? ld1 : 1, cs1 = ctr ? 1 : ld1;
     const long rs2 = ctr ? ld2 : 1, cs2 = ctr ? 1 : ld2;
     const int nstrips = ntiles * NB;
-    for (int sidx = w; sidx < nstrips; sidx += NWAVES) {
+    for (int sidx = w; sidx < nstrips; sidx += NT / 64) {
         const int tile = sidx / NB;
         const int col = (sidx % NB) * 16 + il;
         float *c1 = C1 + (long)tile * s1 + (long)col * cs1 + (long)(4 * q) * rs1;
@@ -733,6 +733,227 @@ __device__ __forceinline__ void update_body_split(const float *__restrict__ V, l
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Update, blocked-staging form (the stream-ordered executor and the operator
+// API): the same compact-WY arithmetic as update_body, with
+//  - only the 16 reflectors of block J in LDS (B x 16, double-buffered; the
+//    next block's global loads are in flight while block J is computed), so a
+//    workgroup needs ~2*B*17 + 16*B elements of LDS and two workgroups share a CU;
+//  - each wave owning two 16-column strips and interleaving them, so every
+//    A-operand read from LDS feeds two MFMAs and the two accumulation chains
+//    are independent.
+// Handles up to 2*NT/64 strips (ntiles * B/16 of them) per call.
+// ---------------------------------------------------------------------------
+constexpr int UPD_THREADS = 256;
+template <typename TS, int B> struct Lds2 {
+    static constexpr int YS = 17; // row stride of a staged 16-reflector block
+    static constexpr int upd = (2 * B * YS * (int)sizeof(TS) + 16 * B * (int)sizeof(TS) + 7) / 8;
+};
+
+template <typename TS, int B, bool APPLYT, int NT, bool CTR>
+__device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V, long ldv, bool vtr,
+                                             const double *__restrict__ tjj, TS *__restrict__ C1, long ld1, long s1,
+                                             TS *__restrict__ C2, long ld2, long s2, int ntiles, double *smem)
+{
+    typedef typename AccT<TS>::type acc_t;
+    constexpr int NB = B / 16, YS = Lds2<TS, B>::YS;
+    // Rows of C (and reflectors of block J) are taken in "memory order": in
+    // step m of a K loop, lane group q supplies row 4q + m, so each lane owns
+    // 4 consecutive rows and the C accesses of the column-major orientation are
+    // 32-byte runs.  The f64 accumulator puts hardware row q + 4m in element m;
+    // the products stay exact because Y's rows, C's rows and the block's
+    // reflector index are permuted consistently: hardware row d <-> memory row
+    // perm(d) = 4*(d % 4) + d / 4 (the identity for the f32 layout, whose
+    // hardware row already is 4q + m).
+    auto perm = [](int d) { return sizeof(TS) == 8 ? 4 * (d & 3) + (d >> 2) : d; };
+    constexpr int PER = (B * 16) / NT; // staged block elements per thread
+    static_assert((B * 16) % NT == 0 && PER >= 1, "update2 staging split");
+    TS *YB = reinterpret_cast<TS *>(smem); // YB[buf*B*YS + k*YS + i] = Y[k][16J + i]
+    TS *TL = YB + 2 * B * YS;              // TL[i*16 + jl] = T[i][16*(i/16)+jl]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, q = lane >> 4, il = lane & 15;
+
+    // (k, i) of the u-th element this thread stages; coalesced along the
+    // contiguous direction of V
+    auto kidx = [&](int u, int &k, int &i) {
+        const int idx = t + NT * u;
+        if (vtr) { i = idx % 16; k = idx / 16; } else { k = idx % B; i = idx / B; }
+    };
+    auto yload = [&](int J, TS(&r)[PER]) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            int k, i;
+            kidx(u, k, i);
+            const int ic = 16 * J + i;
+            TS v = V[toff(k, ic, ldv, vtr)];
+            if (mode == MODE_GE) v = (k < ic) ? (TS)0 : (k == ic ? (TS)1 : v);
+            if (mode == MODE_TT && k > ic) v = (TS)0;
+            r[u] = v;
+        }
+    };
+    auto ystore = [&](int buf, const TS(&r)[PER]) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            int k, i;
+            kidx(u, k, i);
+            YB[buf * B * YS + k * YS + i] = r[u];
+        }
+    };
+    for (int idx = t; idx < 16 * B; idx += NT) TL[idx] = (TS)tjj[idx];
+    {
+        TS r[PER];
+        yload(APPLYT ? 0 : NB - 1, r);
+        ystore(0, r);
+    }
+
+    // logical element (row, col) of a C tile is at base + row*rs + col*cs
+    const long rs1 = CTR ? ld1 : 1, cs1 = CTR ? 1 : ld1;
+    const long rs2 = CTR ? ld2 : 1, cs2 = CTR ? 1 : ld2;
+    const int pil = perm(il);
+    const int nstrips = ntiles * NB;
+    const bool mine = 2 * w < nstrips; // wave-uniform; strips come in pairs (NB is even)
+    TS *c1p[2], *c2p[2];
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp) {
+        const int strip = 2 * w + sp;
+        const int tile = strip / NB, col = (strip % NB) * 16 + il;
+        c1p[sp] = C1 + (long)tile * s1 + (long)col * cs1 + (long)(4 * q) * rs1;
+        c2p[sp] = C2 + (long)tile * s2 + (long)col * cs2 + (long)(4 * q) * rs2;
+    }
+    acc_t acc2[2][NB];
+    if (mine) {
+#pragma unroll
+        for (int sp = 0; sp < 2; ++sp) {
+#pragma unroll
+            for (int tb = 0; tb < NB; ++tb) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) acc2[sp][tb][m] = c2p[sp][(long)(16 * tb + m) * rs2];
+            }
+        }
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int jj = 0; jj < NB; ++jj) {
+        const int J = APPLYT ? jj : (NB - 1 - jj);
+        const TS *YJ = YB + (jj & 1) * B * YS;
+        TS yr[PER];
+        if (jj + 1 < NB) yload(APPLYT ? jj + 1 : NB - 2 - jj, yr);
+        if (mine) {
+            // C1[J] is requested first and added after the product, so its
+            // load overlaps the W MFMAs
+            acc_t c1v[2], Wa[2];
+#pragma unroll
+            for (int sp = 0; sp < 2; ++sp) {
+                Wa[sp] = vzero4<acc_t>();
+                if (mode != MODE_GE) {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) c1v[sp][m] = c1p[sp][(long)(16 * J + m) * rs1];
+                }
+            }
+            // W = Y_J^T C2 + C1[J]
+#pragma unroll
+            for (int tb = 0; tb < NB; ++tb) {
+                if (mode == MODE_GE && tb < J) continue;
+                if (mode == MODE_TT && tb > J) continue;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const TS av = YJ[(16 * tb + 4 * q + m) * YS + pil];
+                    Wa[0] = mfma16(av, acc2[0][tb][m], Wa[0]);
+                    Wa[1] = mfma16(av, acc2[1][tb][m], Wa[1]);
+                }
+            }
+            if (mode != MODE_GE) {
+#pragma unroll
+                for (int sp = 0; sp < 2; ++sp) {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) Wa[sp][m] += c1v[sp][m];
+                }
+            }
+            // W' = T_JJ^T W (Q^T) or T_JJ W (Q)
+            acc_t Wp[2] = {vzero4<acc_t>(), vzero4<acc_t>()};
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const TS av = APPLYT ? TL[(16 * J + 4 * q + m) * 16 + pil] : TL[(16 * J + pil) * 16 + 4 * q + m];
+                Wp[0] = mfma16(av, Wa[0][m], Wp[0]);
+                Wp[1] = mfma16(av, Wa[1][m], Wp[1]);
+            }
+            if (mode != MODE_GE) {
+#pragma unroll
+                for (int sp = 0; sp < 2; ++sp) {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) c1p[sp][(long)(16 * J + m) * rs1] = c1v[sp][m] - Wp[sp][m];
+                }
+            }
+            // C2 -= Y_J W'
+#pragma unroll
+            for (int tb = 0; tb < NB; ++tb) {
+                if (mode == MODE_GE && tb < J) continue;
+                if (mode == MODE_TT && tb > J) continue;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const TS av = -YJ[(16 * tb + pil) * YS + 4 * q + m];
+                    acc2[0][tb] = mfma16(av, Wp[0][m], acc2[0][tb]);
+                    acc2[1][tb] = mfma16(av, Wp[1][m], acc2[1][tb]);
+                }
+            }
+        }
+        if (jj + 1 < NB) ystore((jj + 1) & 1, yr);
+        __syncthreads();
+    }
+    if (mine) {
+#pragma unroll
+        for (int sp = 0; sp < 2; ++sp) {
+#pragma unroll
+            for (int tb = 0; tb < NB; ++tb) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) c2p[sp][(long)(16 * tb + m) * rs2] = acc2[sp][tb][m];
+            }
+        }
+    }
+}
+
+// blocked-staging update with the arithmetic fixed at compile time: split-BF16
+// (FP32 storage only) or the native MFMA of the storage type
+template <typename TS, int B, int MODE, bool APPLYT, bool SPLIT>
+__device__ __forceinline__ void update_fixed(const TS *V, long ldv, bool vtr, const double *tjj, TS *C1, long ld1,
+                                             long s1, TS *C2, long ld2, long s2, bool ctr, int ntiles, double *smem)
+{
+    if constexpr (SPLIT && sizeof(TS) == 4) {
+        update_body_split<B, MODE, APPLYT, UPD_THREADS>((const float *)V, ldv, vtr, tjj, (float *)C1, ld1, s1,
+                                                         (float *)C2, ld2, s2, ctr, ntiles, smem);
+    } else {
+        if (ctr)
+            update2_body<TS, B, APPLYT, UPD_THREADS, true>(MODE, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ntiles, smem);
+        else
+            update2_body<TS, B, APPLYT, UPD_THREADS, false>(MODE, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ntiles,
+                                                            smem);
+    }
+}
+// the same with the tile mode chosen at run time (one body for all modes) and
+// the orientation fixed (V and C both in adjoint addressing when TR)
+template <typename TS, int B, bool SPLIT, bool TR>
+__device__ __forceinline__ void update_any(int mode, const TS *V, long ldv, const double *tjj, TS *C1, long ld1,
+                                           long s1, TS *C2, long ld2, long s2, int ntiles, double *smem)
+{
+    const bool vtr = TR, ctr = TR;
+    if constexpr (SPLIT && sizeof(TS) == 4) {
+        if (mode == MODE_GE)
+            update_fixed<TS, B, MODE_GE, true, true>(V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, smem);
+        else if (mode == MODE_TS)
+            update_fixed<TS, B, MODE_TS, true, true>(V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, smem);
+        else
+            update_fixed<TS, B, MODE_TT, true, true>(V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ctr, ntiles, smem);
+    } else {
+        update2_body<TS, B, true, UPD_THREADS, TR>(mode, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ntiles, smem);
+    }
+}
+// LDS (doubles) of update_fixed
+template <typename TS, int B, bool SPLIT> struct LdsU {
+    static constexpr int value = (SPLIT && sizeof(TS) == 4) ? Lds<TS, B>::update_split : Lds2<TS, B>::upd;
+};
+// column tiles one update workgroup handles (8 strips of 16 columns)
+template <int B> struct UpdTiles { static constexpr int value = (2 * UPD_THREADS / 64) / (B / 16); };
 
 // update with the arithmetic selected at run time: split-BF16 (FP32 storage
 // only) or the native MFMA of the storage type

@@ -79,24 +79,20 @@ __global__ void __launch_bounds__(NTHREADS) tjj_kernel(const TS *V, long ldv, in
     tjj_body<TS, B, MODE>(V, ldv, vtr != 0, tau, tjj, smem);
 }
 
-// one block per group of `ntiles` column tiles; group g starts at tile g*ntiles
-template <typename TS, int B, int MODE, bool APPLYT>
-__global__ void __launch_bounds__(NTHREADS) update_kernel(const TS *V, long ldv, int vtr, const double *tjj,
-                                                          TS *C1, long ld1, long s1, TS *C2, long ld2, long s2,
-                                                          int ctr, int ntiles_total, int per_block, int split)
+// one block per group of `per_block` column tiles (at most UpdTiles<B>);
+// group g starts at tile g*per_block
+template <typename TS, int B, int MODE, bool APPLYT, bool SPLIT>
+__global__ void __launch_bounds__(UPD_THREADS, 2) update_kernel(const TS *V, long ldv, int vtr, const double *tjj,
+                                                             TS *C1, long ld1, long s1, TS *C2, long ld2, long s2,
+                                                             int ctr, int ntiles_total, int per_block)
 {
-    __shared__ double smem[Lds<TS, B>::upd];
+    __shared__ double smem[LdsU<TS, B, SPLIT>::value];
     const int first = blockIdx.x * per_block;
     int cnt = ntiles_total - first;
     if (cnt > per_block) cnt = per_block;
     if (cnt <= 0) return;
-    if constexpr (APPLYT) {
-        update_sel<TS, B, MODE>(split != 0, V, ldv, vtr != 0, tjj, C1 + (long)first * s1, ld1, s1,
-                                C2 + (long)first * s2, ld2, s2, ctr != 0, cnt, smem);
-    } else {
-        update_body<TS, B, MODE, APPLYT>(V, ldv, vtr != 0, tjj, C1 + (long)first * s1, ld1, s1, C2 + (long)first * s2,
-                                         ld2, s2, ctr != 0, cnt, smem);
-    }
+    update_fixed<TS, B, MODE, APPLYT, SPLIT>(V, ldv, vtr != 0, tjj, C1 + (long)first * s1, ld1, s1,
+                                             C2 + (long)first * s2, ld2, s2, ctr != 0, cnt, smem);
 }
 
 // ------------------------------------------------------ persistent executor
@@ -192,7 +188,7 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
 // ------------------------------------------------ stream-ordered executor
 // Panel phase: workgroup c runs the panel operations of chain c in order
 // (a domain's GEQRT + TSQRT chain, or one tree merge).
-template <typename TS, int B>
+template <typename TS, int B, int MODE>
 __global__ void __launch_bounds__(PanelThreads<B>::value) chain_panel_kernel(TS *tiles, double *taus, double *tjjs,
                                                                              const ChainOp *__restrict__ ops,
                                                                              const int *__restrict__ chain_start,
@@ -208,10 +204,7 @@ __global__ void __launch_bounds__(PanelThreads<B>::value) chain_panel_kernel(TS 
         TS *bot = tiles + (long)T.btile * tsz;
         double *tau = taus + (long)T.fac * B;
         double *tjj = tjjs + (long)T.fac * 16 * B;
-        const bool tr = T.trans != 0;
-        if (T.mode == MODE_GE) panel_body<TS, B, MODE_GE, NT>(top, B, top, B, tr, tau, tjj, smem);
-        else if (T.mode == MODE_TS) panel_body<TS, B, MODE_TS, NT>(top, B, bot, B, tr, tau, tjj, smem);
-        else panel_body<TS, B, MODE_TT, NT>(top, B, bot, B, tr, tau, tjj, smem);
+        panel_body<TS, B, MODE, NT>(top, B, MODE == MODE_GE ? top : bot, B, T.trans != 0, tau, tjj, smem);
         __syncthreads();
     }
 }
@@ -219,13 +212,13 @@ __global__ void __launch_bounds__(PanelThreads<B>::value) chain_panel_kernel(TS 
 // Update phase: workgroup (chain c, column group g) applies every operation of
 // chain c, in order, to the `per` column tiles of group g.  The chain's C1 row
 // tile is rewritten by each operation and stays hot in L2.
-template <typename TS, int B>
-__global__ void __launch_bounds__(NTHREADS) chain_update_kernel(TS *tiles, const double *tjjs,
-                                                                const ChainOp *__restrict__ ops,
-                                                                const int *__restrict__ chain_start, int chain0,
-                                                                int groups, int per, int split)
+template <typename TS, int B, bool SPLIT, bool TR, int MODE>
+__global__ void __launch_bounds__(UPD_THREADS, 2) chain_update_kernel(TS *tiles, const double *tjjs,
+                                                                       const ChainOp *__restrict__ ops,
+                                                                       const int *__restrict__ chain_start, int chain0,
+                                                                       int groups, int per)
 {
-    __shared__ double smem[Lds<TS, B>::upd];
+    __shared__ double smem[LdsU<TS, B, SPLIT>::value];
     const long tsz = (long)B * B;
     const int c = chain0 + blockIdx.x / groups;
     const int g = blockIdx.x % groups;
@@ -237,32 +230,48 @@ __global__ void __launch_bounds__(NTHREADS) chain_update_kernel(TS *tiles, const
         const TS *V = tiles + (long)T.vtile * tsz;
         const long st = (long)T.cstep * tsz;
         TS *C2 = tiles + (long)T.btile * tsz + (long)first * st;
-        TS *C1 = tiles + (long)T.c1tile * tsz + (long)first * st;
+        // GE (UNMQR) updates the one tile it is given: C1 = C2
+        TS *C1 = MODE == MODE_GE ? C2 : tiles + (long)T.c1tile * tsz + (long)first * st;
         const double *tjj = tjjs + (long)T.fac * 16 * B;
-        const bool tr = T.trans != 0;
-        if (T.mode == MODE_GE)
-            update_sel<TS, B, MODE_GE>(split != 0, V, B, tr, tjj, C2, B, st, C2, B, st, tr, cnt, smem);
-        else if (T.mode == MODE_TS)
-            update_sel<TS, B, MODE_TS>(split != 0, V, B, tr, tjj, C1, B, st, C2, B, st, tr, cnt, smem);
-        else
-            update_sel<TS, B, MODE_TT>(split != 0, V, B, tr, tjj, C1, B, st, C2, B, st, tr, cnt, smem);
+        update_fixed<TS, B, MODE, true, SPLIT>(V, B, TR, tjj, C1, B, st, C2, B, st, TR, cnt, smem);
         __syncthreads();
     }
 }
 
 template <typename TS, int B>
-void launch_chain_panel(int nchains, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
+void launch_chain_panel(int nchains, int mode, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
                         const int *chain_start, int chain0, hipStream_t s)
 {
-    hipLaunchKernelGGL((chain_panel_kernel<TS, B>), dim3(nchains), dim3(PanelThreads<B>::value), 0, s, (TS *)tiles, taus, tjjs, ops,
-                       chain_start, chain0);
+#define AVS_CHP(M)                                                                                                     \
+    hipLaunchKernelGGL((chain_panel_kernel<TS, B, M>), dim3(nchains), dim3(PanelThreads<B>::value), 0, s, (TS *)tiles, \
+                       taus, tjjs, ops, chain_start, chain0)
+    if (mode == MODE_GE) AVS_CHP(MODE_GE);
+    else if (mode == MODE_TS) AVS_CHP(MODE_TS);
+    else AVS_CHP(MODE_TT);
+#undef AVS_CHP
 }
 template <typename TS, int B>
-void launch_chain_update(int nchains, int groups, int per, void *tiles, const double *tjjs, const ChainOp *ops,
-                         const int *chain_start, int chain0, int split, hipStream_t s)
+void launch_chain_update(int nchains, int groups, int per, int mode, int trans, void *tiles, const double *tjjs,
+                         const ChainOp *ops, const int *chain_start, int chain0, int split, hipStream_t s)
 {
-    hipLaunchKernelGGL((chain_update_kernel<TS, B>), dim3(nchains * groups), dim3(NTHREADS), 0, s, (TS *)tiles, tjjs,
-                       ops, chain_start, chain0, groups, per, split);
+#define AVS_CHU(SP, TR, M)                                                                                             \
+    hipLaunchKernelGGL((chain_update_kernel<TS, B, SP, TR, M>), dim3(nchains * groups), dim3(UPD_THREADS), 0, s,      \
+                       (TS *)tiles, tjjs, ops, chain_start, chain0, groups, per)
+#define AVS_CHU_M(SP, TR)                                                                                              \
+    if (mode == MODE_GE) AVS_CHU(SP, TR, MODE_GE);                                                                     \
+    else if (mode == MODE_TS) AVS_CHU(SP, TR, MODE_TS);                                                                \
+    else AVS_CHU(SP, TR, MODE_TT)
+    if constexpr (sizeof(TS) == 4) {
+        if (split) {
+            if (trans) { AVS_CHU_M(true, true); }
+            else { AVS_CHU_M(true, false); }
+            return;
+        }
+    }
+    if (trans) { AVS_CHU_M(false, true); }
+    else { AVS_CHU_M(false, false); }
+#undef AVS_CHU_M
+#undef AVS_CHU
 }
 
 // ---------------------------------------------------------- batched form
@@ -272,6 +281,33 @@ void launch_chain_update(int nchains, int groups, int per, void *tiles, const do
 // Blockbidiag! schedule on it (tiledalgos.jl:17-34) with the same tile
 // operators as the single-matrix path; all reflector factors of the matrix
 // being reduced live in a per-workgroup scratch slot.
+// update of nc column tiles by a whole workgroup of NTHREADS, with the
+// per-strip arithmetic of the single-matrix executors (so a matrix reduced in
+// a batch gets the same bits as on its own)
+template <typename TS, int B, int MODE>
+__device__ __forceinline__ void batch_update(int split, const TS *V, bool vtr, const double *tjj, TS *C1, long s1,
+                                             TS *C2, long s2, bool ctr, int nc, double *smem)
+{
+    if constexpr (sizeof(TS) == 4) {
+        if (split) {
+            update_body_split<B, MODE, true, NTHREADS>((const float *)V, B, vtr, tjj, (float *)C1, B, s1, (float *)C2,
+                                                       B, s2, ctr, nc, smem);
+            return;
+        }
+    }
+    constexpr int CHUNK = (2 * NWAVES) / (B / 16); // column tiles per pass (16 strips)
+    for (int c0 = 0; c0 < nc; c0 += CHUNK) {
+        const int cnt = min(CHUNK, nc - c0);
+        if (ctr)
+            update2_body<TS, B, true, NTHREADS, true>(MODE, V, B, vtr, tjj, C1 + (long)c0 * s1, B, s1,
+                                                      C2 + (long)c0 * s2, B, s2, cnt, smem);
+        else
+            update2_body<TS, B, true, NTHREADS, false>(MODE, V, B, vtr, tjj, C1 + (long)c0 * s1, B, s1,
+                                                       C2 + (long)c0 * s2, B, s2, cnt, smem);
+        __syncthreads();
+    }
+}
+
 template <typename TS, int B>
 __global__ void __launch_bounds__(NTHREADS, 1) batch_kernel(TS *arena, const long *__restrict__ off,
                                                              const int *__restrict__ nts,
@@ -299,29 +335,29 @@ __global__ void __launch_bounds__(NTHREADS, 1) batch_kernel(TS *arena, const lon
             panel_body<TS, B, MODE_GE>(tile(k, k), B, tile(k, k), B, false, tau, tjj, smem);
             __syncthreads();
             if (nc > 0)
-                update_sel<TS, B, MODE_GE>(split, tile(k, k), B, false, tjj, tile(k, k + 1), B, nt * tsz,
-                                           tile(k, k + 1), B, nt * tsz, false, nc, smem);
+                batch_update<TS, B, MODE_GE>(split, tile(k, k), false, tjj, tile(k, k + 1), nt * tsz, tile(k, k + 1),
+                                             nt * tsz, false, nc, smem);
             __syncthreads();
             for (int i = k + 1; i < nt; ++i) {
                 panel_body<TS, B, MODE_TS>(tile(k, k), B, tile(i, k), B, false, tau, tjj, smem);
                 __syncthreads();
                 if (nc > 0)
-                    update_sel<TS, B, MODE_TS>(split, tile(i, k), B, false, tjj, tile(k, k + 1), B, nt * tsz,
-                                               tile(i, k + 1), B, nt * tsz, false, nc, smem);
+                    batch_update<TS, B, MODE_TS>(split, tile(i, k), false, tjj, tile(k, k + 1), nt * tsz,
+                                                 tile(i, k + 1), nt * tsz, false, nc, smem);
                 __syncthreads();
             }
             if (k == nt - 1) break;
             // LQ sweep on tile row k from column k+1 (adjoint addressing)
             panel_body<TS, B, MODE_GE>(tile(k, k + 1), B, tile(k, k + 1), B, true, tau, tjj, smem);
             __syncthreads();
-            update_sel<TS, B, MODE_GE>(split, tile(k, k + 1), B, true, tjj, tile(k + 1, k + 1), B, tsz,
-                                       tile(k + 1, k + 1), B, tsz, true, nc, smem);
+            batch_update<TS, B, MODE_GE>(split, tile(k, k + 1), true, tjj, tile(k + 1, k + 1), tsz, tile(k + 1, k + 1),
+                                         tsz, true, nc, smem);
             __syncthreads();
             for (int j = k + 2; j < nt; ++j) {
                 panel_body<TS, B, MODE_TS>(tile(k, k + 1), B, tile(k, j), B, true, tau, tjj, smem);
                 __syncthreads();
-                update_sel<TS, B, MODE_TS>(split, tile(k, j), B, true, tjj, tile(k + 1, k + 1), B, tsz,
-                                           tile(k + 1, j), B, tsz, true, nc, smem);
+                batch_update<TS, B, MODE_TS>(split, tile(k, j), true, tjj, tile(k + 1, k + 1), tsz, tile(k + 1, j),
+                                             tsz, true, nc, smem);
                 __syncthreads();
             }
         }
@@ -488,14 +524,21 @@ static void launch_update_m(bool applyt, const void *V, long ldv, int vtr, const
                             long s1, void *C2, long ld2, long s2, int ctr, int ntiles, int per_block, int split,
                             hipStream_t s)
 {
+    if (per_block > UpdTiles<B>::value || per_block < 1) per_block = UpdTiles<B>::value;
     const int nblk = (ntiles + per_block - 1) / per_block;
     if (nblk <= 0) return;
-    if (applyt)
-        hipLaunchKernelGGL((update_kernel<TS, B, MODE, true>), dim3(nblk), dim3(NTHREADS), 0, s, (const TS *)V, ldv,
-                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block, split);
-    else
-        hipLaunchKernelGGL((update_kernel<TS, B, MODE, false>), dim3(nblk), dim3(NTHREADS), 0, s, (const TS *)V, ldv,
-                           vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block, 0);
+#define AVS_UPD(AT, SP)                                                                                                \
+    hipLaunchKernelGGL((update_kernel<TS, B, MODE, AT, SP>), dim3(nblk), dim3(UPD_THREADS), 0, s, (const TS *)V, ldv,  \
+                       vtr, tjj, (TS *)C1, ld1, s1, (TS *)C2, ld2, s2, ctr, ntiles, per_block)
+    const bool sp = sizeof(TS) == 4 && split;
+    if (applyt) {
+        if (sp) AVS_UPD(true, true);
+        else AVS_UPD(true, false);
+    } else {
+        if (sp) AVS_UPD(false, true);
+        else AVS_UPD(false, false);
+    }
+#undef AVS_UPD
 }
 
 template <typename TS, int B>
@@ -548,10 +591,10 @@ void launch_mfma_probe(const double *a, const double *b, double *o64, const floa
     template void launch_batch<TS, B>(int, void *, const long *, const int *, const int *, int, double *, int *, int, \
                                       hipStream_t);                                                                    \
     template int batch_occupancy<TS, B>();                                                                             \
-    template void launch_chain_panel<TS, B>(int, void *, double *, double *, const ChainOp *, const int *, int,        \
+    template void launch_chain_panel<TS, B>(int, int, void *, double *, double *, const ChainOp *, const int *, int,   \
                                             hipStream_t);                                                              \
-    template void launch_chain_update<TS, B>(int, int, int, void *, const double *, const ChainOp *, const int *, int, \
-                                             int, hipStream_t);
+    template void launch_chain_update<TS, B>(int, int, int, int, int, void *, const double *, const ChainOp *,         \
+                                             const int *, int, int, hipStream_t);
 
 AVS_INST_B(double, 32)
 AVS_INST_B(double, 64)

@@ -40,11 +40,11 @@ void launch_batch_band(const void *arena, const long *off, const int *nts, const
                        double *bands, const long *boff, int count, int b, hipStream_t s);
 
 template <typename TS, int B>
-void launch_chain_panel(int nchains, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
+void launch_chain_panel(int nchains, int mode, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
                         const int *chain_start, int chain0, hipStream_t s);
 template <typename TS, int B>
-void launch_chain_update(int nchains, int groups, int per, void *tiles, const double *tjjs, const ChainOp *ops,
-                         const int *chain_start, int chain0, int split, hipStream_t s);
+void launch_chain_update(int nchains, int groups, int per, int mode, int trans, void *tiles, const double *tjjs,
+                         const ChainOp *ops, const int *chain_start, int chain0, int split, hipStream_t s);
 
 void launch_mfma_probe(const double *a, const double *b, double *o64, const float *af, const float *bf, float *o32,
                        int lanes, hipStream_t s);

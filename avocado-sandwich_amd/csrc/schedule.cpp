@@ -253,9 +253,10 @@ struct PhaseBuilder {
     explicit PhaseBuilder(PhaseProgram &p, int nt_) : P(p), nt(nt_) {}
     int tid(int i, int j) const { return j * nt + i; }
     // opens a phase; chains are appended with chain()
-    void begin(int kind, int ncols)
+    int cur_trans = 0;
+    void begin(int kind, int ncols, int mode)
     {
-        Phase ph{kind, (int)P.chain_start.size() - 1, 0, ncols};
+        Phase ph{kind, (int)P.chain_start.size() - 1, 0, ncols, cur_trans, mode};
         P.phases.push_back(ph);
     }
     void chain(const std::vector<ChainOp> &ops)
@@ -293,6 +294,7 @@ struct PhaseBuilder {
     {
         const int m = lq ? nt - k - 1 : nt - k;
         if (m <= 0) return;
+        cur_trans = lq ? 1 : 0;
         auto T = [&](int p) { return lq ? tid(k, k + 1 + p) : tid(k + p, k); };
         auto U0 = [&](int p) { return lq ? tid(k + 1, k + 1 + p) : tid(k + p, k + 1); };
         const int step = lq ? 1 : nt;
@@ -302,24 +304,34 @@ struct PhaseBuilder {
         const int h = (tree == 0 || domain <= 0) ? m : domain;
         std::vector<int> heads;
         for (int d0 = 0; d0 < m; d0 += h) heads.push_back(d0);
-        // domain panel chains
-        begin(TK_PANEL, 0);
-        for (int d0 : heads) {
-            std::vector<ChainOp> c;
-            const int d1 = std::min(d0 + h, m);
-            c.push_back(panel(MODE_GE, lq, T(d0), T(d0), famGE * nt2 + T(d0)));
-            for (int p = d0 + 1; p < d1; ++p) c.push_back(panel(MODE_TS, lq, T(d0), T(p), famGE * nt2 + T(p)));
-            chain(c);
-        }
+        // GEQRT of every domain head, then its update
+        begin(TK_PANEL, 0, MODE_GE);
+        for (int d0 : heads) chain({panel(MODE_GE, lq, T(d0), T(d0), famGE * nt2 + T(d0))});
         if (cnt > 0) {
-            begin(TK_UPDATE, cnt);
+            begin(TK_UPDATE, cnt, MODE_GE);
+            for (int d0 : heads) chain({update(MODE_GE, lq, T(d0), U0(d0), U0(d0), step, cnt, famGE * nt2 + T(d0))});
+        }
+        // the TSQRT chain of every domain (on its head's R), then its updates
+        if (h > 1) {
+            begin(TK_PANEL, 0, MODE_TS);
             for (int d0 : heads) {
-                std::vector<ChainOp> c;
                 const int d1 = std::min(d0 + h, m);
-                c.push_back(update(MODE_GE, lq, T(d0), U0(d0), U0(d0), step, cnt, famGE * nt2 + T(d0)));
-                for (int p = d0 + 1; p < d1; ++p)
-                    c.push_back(update(MODE_TS, lq, T(p), U0(d0), U0(p), step, cnt, famGE * nt2 + T(p)));
+                if (d1 - d0 < 2) continue;
+                std::vector<ChainOp> c;
+                for (int p = d0 + 1; p < d1; ++p) c.push_back(panel(MODE_TS, lq, T(d0), T(p), famGE * nt2 + T(p)));
                 chain(c);
+            }
+            if (P.phases.back().nchains == 0) P.phases.pop_back();
+            else if (cnt > 0) {
+                begin(TK_UPDATE, cnt, MODE_TS);
+                for (int d0 : heads) {
+                    const int d1 = std::min(d0 + h, m);
+                    if (d1 - d0 < 2) continue;
+                    std::vector<ChainOp> c;
+                    for (int p = d0 + 1; p < d1; ++p)
+                        c.push_back(update(MODE_TS, lq, T(p), U0(d0), U0(p), step, cnt, famGE * nt2 + T(p)));
+                    chain(c);
+                }
             }
         }
         // pairwise merges of the domain heads, one level per phase pair
@@ -327,10 +339,10 @@ struct PhaseBuilder {
         for (int s2 = 1; s2 < nh; s2 *= 2) {
             std::vector<std::pair<int, int>> pairs;
             for (int a = 0; a + s2 < nh; a += 2 * s2) pairs.push_back({heads[a], heads[a + s2]});
-            begin(TK_PANEL, 0);
+            begin(TK_PANEL, 0, MODE_TT);
             for (auto &pr : pairs) chain({panel(MODE_TT, lq, T(pr.first), T(pr.second), famTT * nt2 + T(pr.second))});
             if (cnt > 0) {
-                begin(TK_UPDATE, cnt);
+                begin(TK_UPDATE, cnt, MODE_TT);
                 for (auto &pr : pairs)
                     chain({update(MODE_TT, lq, T(pr.second), U0(pr.first), U0(pr.second), step, cnt,
                                   famTT * nt2 + T(pr.second))});

@@ -19,13 +19,18 @@ struct Program {
 };
 
 // Stream-ordered form of the same reduction: a sequence of phases, each one
-// kernel launch whose workgroups run independent chains of operations
-// (a chain runs sequentially inside one workgroup).  Panel phases: one
-// workgroup per chain.  Update phases: one workgroup per (chain, column group).
+// kernel launch whose workgroups run independent chains of operations of a
+// single tile mode (a chain runs sequentially inside one workgroup).  Panel
+// phases: one workgroup per chain.  Update phases: one workgroup per (chain,
+// column group).  Per sweep: the GEQRT of every domain head, the TSQRT chain
+// of every domain, then one TT merge level after another; each panel phase
+// is followed by the update phase of the same operations.
 struct Phase {
     int kind;         // TK_PANEL or TK_UPDATE
     int chain0, nchains;
     int ncols;        // update: column tiles of every op in the phase
+    int trans;        // every op of the phase is QR (0) or LQ (1)
+    int mode;         // every op of the phase has this MODE_* (one kernel body per launch)
 };
 struct PhaseProgram {
     int nt = 0, B = 0, alg = 0, kend = 0, tree = 0, domain = 0;

@@ -27,7 +27,8 @@ def compile_kernels(tilesize=32, tilefactor=1, tiledim2d=False):
     _compiled.update(tilesize=tilesize, tilefactor=tilefactor, tiledim2d=tiledim2d)
 
 
-def _opts(sched=None, tree=None, domain=0, update_mode=0, workers=0, wait_ms=0):
+def _opts(sched=None, tree=None, domain=0, update_mode=0, workers=0, wait_ms=0, single_stream=False,
+          phase_timing=False):
     o = Opts()
     lib().avs_default_opts(ctypes.byref(o))
     if sched is not None:
@@ -38,6 +39,8 @@ def _opts(sched=None, tree=None, domain=0, update_mode=0, workers=0, wait_ms=0):
     o.update_mode = update_mode
     o.workers = workers
     o.reserved[0] = wait_ms
+    o.reserved[1] = 1 if phase_timing else 0
+    o.reserved[2] = 1 if single_stream else 0
     return o
 
 

@@ -132,6 +132,10 @@ struct avs_handle {
     std::vector<int> pkind;
     std::vector<double> pflops;
     int plaunches = 0;
+    // panel stream of the stream-ordered executor and one completion event per
+    // phase (the panels of a sweep run ahead of that sweep's updates)
+    hipStream_t pstream = nullptr;
+    std::vector<hipEvent_t> done;
 };
 
 namespace {
@@ -257,38 +261,73 @@ double phase_update_flops(const PhaseProgram &P, const Phase &ph)
     return f;
 }
 
+// Stream-ordered execution.  Updates run in program order on the handle's
+// stream.  Unless opts.reserved[2] is set, panels run in program order on a
+// second stream: within a sweep the panel chain (GEQRT, TSQRT chains, merge
+// levels) does not depend on that sweep's updates, so it runs ahead of them;
+// each update phase waits for its own panel phase, and the first panel phase
+// of a sweep waits for every update before it.  With opts.reserved[1] set,
+// every launch is bracketed by timing events on its stream.
 template <typename TS, int B> avs_status run_streams(avs_handle *h)
 {
     const DevPhases &D = *h->phases;
     const int per = upd_tiles_for(B);
     const bool timing = h->opts.reserved[1] != 0;
+    const bool two = h->opts.reserved[2] == 0;
+    const size_t nph = D.P.phases.size();
     h->plaunches = 0;
     if (timing) {
-        const size_t need = D.P.phases.size() + 1;
-        while (h->pev.size() < need) {
+        while (h->pev.size() < 2 * nph) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
             h->pev.push_back(e);
         }
-        h->pkind.assign(D.P.phases.size(), 0);
-        h->pflops.assign(D.P.phases.size(), 0.0);
-        HIPCHK(hipEventRecord(h->pev[0], h->stream));
+        h->pkind.assign(nph, 0);
+        h->pflops.assign(nph, 0.0);
     }
-    for (const Phase &ph : D.P.phases) {
+    if (two) {
+        if (!h->pstream) HIPCHK(hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking));
+        while (h->done.size() < nph + 1) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            h->done.push_back(e);
+        }
+    }
+    hipStream_t us = h->stream, ps = two ? h->pstream : h->stream;
+    // done[nph]: the latest update (or the matrix upload) the next sweep's panels wait for
+    hipEvent_t last_update = two ? h->done[nph] : nullptr;
+    if (two) HIPCHK(hipEventRecord(last_update, us));
+    int last_panel = -1;
+    for (size_t i = 0; i < nph; ++i) {
+        const Phase &ph = D.P.phases[i];
+        hipStream_t st = ph.kind == TK_PANEL ? ps : us;
+        if (two) {
+            if (ph.kind == TK_PANEL && ph.mode == MODE_GE) HIPCHK(hipStreamWaitEvent(ps, last_update, 0));
+            if (ph.kind == TK_UPDATE && last_panel >= 0) HIPCHK(hipStreamWaitEvent(us, h->done[last_panel], 0));
+        }
         if (timing) {
-            h->pkind[h->plaunches] = ph.kind;
-            h->pflops[h->plaunches] = ph.kind == TK_UPDATE ? phase_update_flops(D.P, ph) : 0.0;
+            h->pkind[i] = ph.kind;
+            h->pflops[i] = ph.kind == TK_UPDATE ? phase_update_flops(D.P, ph) : 0.0;
+            HIPCHK(hipEventRecord(h->pev[2 * i], st));
         }
         if (ph.kind == TK_PANEL) {
             launch_chain_panel<TS, B>(ph.nchains, ph.mode, h->dTiles, h->dTau, h->dTjj, D.dOps, D.dChain, ph.chain0,
-                                      h->stream);
+                                      st);
+            last_panel = (int)i;
         } else {
             const int groups = (ph.ncols + per - 1) / per;
             launch_chain_update<TS, B>(ph.nchains, groups, per, ph.mode, ph.trans, h->dTiles, h->dTjj, D.dOps,
-                                       D.dChain, ph.chain0, split_of(h), h->stream);
+                                       D.dChain, ph.chain0, split_of(h), st);
         }
-        if (timing) HIPCHK(hipEventRecord(h->pev[++h->plaunches], h->stream));
+        if (timing) HIPCHK(hipEventRecord(h->pev[2 * i + 1], st));
+        if (two) {
+            if (ph.kind == TK_PANEL) HIPCHK(hipEventRecord(h->done[i], ps));
+            else HIPCHK(hipEventRecord(last_update, us));
+        }
+        ++h->plaunches;
     }
+    // the handle's stream ends after the last panel too
+    if (two && last_panel >= 0) HIPCHK(hipStreamWaitEvent(us, h->done[last_panel], 0));
     HIPCHK(hipGetLastError());
     return AVS_OK;
 }
@@ -431,6 +470,8 @@ avs_status avs_destroy(avs_handle *h)
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     for (hipEvent_t e : h->pev) hipEventDestroy(e);
+    for (hipEvent_t e : h->done) hipEventDestroy(e);
+    if (h->pstream) hipStreamDestroy(h->pstream);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return AVS_OK;
@@ -538,12 +579,12 @@ avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_laun
     if (!h) return fail(-1, "null handle");
     if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.reserved[1] == 0 || h->plaunches == 0)
         return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.reserved[1] = 1)");
-    HIPCHK(hipEventSynchronize(h->pev[h->plaunches]));
+    HIPCHK(hipStreamSynchronize(h->stream));
     double pm = 0, um = 0, uf = 0;
     int64_t pn = 0, un = 0;
     for (int i = 0; i < h->plaunches; ++i) {
         float f = 0.f;
-        HIPCHK(hipEventElapsedTime(&f, h->pev[i], h->pev[i + 1]));
+        HIPCHK(hipEventElapsedTime(&f, h->pev[2 * i], h->pev[2 * i + 1]));
         if (h->pkind[i] == TK_PANEL) { pm += f; ++pn; }
         else { um += f; ++un; uf += h->pflops[i]; }
     }

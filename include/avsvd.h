@@ -75,7 +75,13 @@ typedef struct avs_opts {
     int32_t workers;       /* persistent workgroups (0 = one per CU) */
     int32_t reserved[9];   /* reserved[0]: bound in ms on any one dependency wait of
                               the persistent executor (0 = 30000); exceeding it ends
-                              the launch and the next sync returns AVS_ERR_STATE */
+                              the launch and the next sync returns AVS_ERR_STATE.
+                              reserved[1] != 0: per-launch timing events for
+                              avs_phase_timing (AVS_SCHED_STREAMS).
+                              reserved[2] != 0: AVS_SCHED_STREAMS runs panels on the
+                              handle's stream too (by default they run on a second
+                              stream of the handle, ahead of the same sweep's updates).
+                              The rest must be 0. */
 } avs_opts;
 
 typedef struct avs_handle avs_handle;
@@ -116,7 +122,9 @@ avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
  * ldband >= b+1.  Only (b+1) x n values cross PCIe, not n x n. */
 /* Per-launch device time of the last avs_blockbidiag / avs_tiled_qr run under
  * AVS_SCHED_STREAMS, for handles created with opts.reserved[1] != 0 (one event
- * pair per launch): summed milliseconds and launch counts of the panel and
+ * pair per launch, on the stream the launch runs on; with the default second
+ * panel stream, panel and update launches overlap in time, so the two sums can
+ * exceed the run's wall time): summed milliseconds and launch counts of the panel and
  * update kernels, and the algorithmic flops of the update launches (4b^3 per
  * TS-operation column tile, 2b^3 per GE/TT-operation column tile).  Waits for
  * the run.  No counterpart in the reference (its benchmarks time whole calls). */

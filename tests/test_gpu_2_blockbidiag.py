@@ -104,6 +104,22 @@ def test_flat_tree_band_matches_oracle(gpu, n, b):
     np.testing.assert_array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("n,b,dt,tree", [(1024, 64, np.float64, avsvd.TREE_HIER), (768, 128, np.float64, avsvd.TREE_HIER),
+                                         (512, 32, np.float32, avsvd.TREE_FLAT)])
+def test_panel_stream_overlap_is_bitwise_neutral(gpu, n, b, dt, tree):
+    """Panels running ahead on their own stream change no bit of the result:
+    every tile sees the same operations in the same order as on one stream."""
+    a = np.asfortranarray(O.gaussian_matrix(n, BASE + 13 * n).astype(dt))
+    outs = []
+    for single in (True, False):
+        print("case n=%d b=%d %s single_stream=%s" % (n, b, np.dtype(dt).name, single), flush=True)
+        A = np.asfortranarray(a.copy())
+        T = avsvd.TiledMatrix(A, b, b, sched=avsvd.SCHED_STREAMS, tree=tree, single_stream=single)
+        avsvd.Blockbidiag_(T)
+        outs.append(T.A.copy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("sched,tree", [(avsvd.SCHED_HOST, avsvd.TREE_FLAT),
                                         (avsvd.SCHED_HOST, avsvd.TREE_HIER),
                                         (avsvd.SCHED_STREAMS, avsvd.TREE_FLAT),

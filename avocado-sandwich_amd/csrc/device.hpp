@@ -79,7 +79,7 @@ template <int B> struct PanelThreads { static constexpr int value = B >= 128 ? 1
 template <typename TS, int B> struct Lds {
     // packed R, broadcast vectors and their norm partials (sized for the widest
     // geometry), Gram rows, tau, beta
-    static constexpr int panel = B * (B + 1) / 2 + 2 * (B + 128) + 2 * 64 + 16 * B + 2 * B;
+    static constexpr int panel = B * (B + 1) / 2 + 2 * (B + 128) + 2 * 64 + 16 * B + 2 * B + 2 * B;
     static constexpr int update = (B * (B + 1) * (int)sizeof(TS) + 16 * B * (int)sizeof(TS) + 7) / 8;
     static constexpr int tjj = 16 * B + B + 16 * B;
     // split-BF16 update (FP32 storage): Y and Y^T as bf16 hi/lo pairs + T_JJ in FP32
@@ -238,6 +238,7 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
     double *gramL = nbuf + 2 * LPC;       // gramL[j*16 + (c&15)] = y_c . y_j
     double *tauL = gramL + 16 * B;        // B
     double *dgL = tauL + B;               // B: R diagonal (beta_j) as produced
+    double *rowb = dgL + B;               // 2 x B (GE): row j of every column before reflector j
     auto rpk = [](int r, int cc) { return r * B - (r * (r - 1)) / 2 + (cc - r); };
     auto xi = [](int row) { return row + (row / RPL) * 2; };
 
@@ -279,6 +280,11 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
         nbuf[(p & 1) * LPC + q] = tree_sum(sq);
     };
     if (c == 0) publish(0);
+    // GE: the lane holding row p of column c hands a_c[p] to reflector p
+    auto pass_row = [&](int p) {
+        if (q == p / RPL) rowb[(p & 1) * B + c] = reg_get(a, p % RPL);
+    };
+    if (MODE == MODE_GE) pass_row(0);
 
     const int jend = (MODE == MODE_GE) ? B - 1 : B;
     for (int j = 0; j < jend; ++j) {
@@ -298,11 +304,13 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
         double nv[LPC];
 #pragma unroll
         for (int u = 0; u < LPC; ++u) nv[u] = nb[u];
-        double xr[RPL], pr[RPL];
+        // GE: a_c[j], the pivot-row entry of this column (R[j][c] once reflector j is applied)
+        const double pj = (MODE == MODE_GE) ? rowb[(j & 1) * B + c] : 0.0;
+        double xr[RPL], Sp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < RPL; ++s) {
             xr[s] = xb[xi(r0 + s)];
-            pr[s] = xr[s] * a[s];
+            Sp[s & 3] = fma(xr[s], a[s], Sp[s & 3]);
         }
         // Householder scalars from ||x||^2 and alpha alone, overlapping the
         // reduction of S.  With rn = 1/nrm:  beta = -sgn(alpha) nrm,
@@ -320,23 +328,20 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
         const double srn = sg * rn;
         const double scal = srn * itau;
         const double c2 = rn * rn * itau;
-        // S_c = x . a_c and (GE) a_c[j]
-        const double S = colsum<CPW>(tree_sum(pr));
+        // S_c = x . a_c
+        const double S = colsum<CPW>((Sp[0] + Sp[1]) + (Sp[2] + Sp[3]));
         AVS_PANEL_MARK(2, j, c == j + 1 && q == 0 && S == S && c2 == c2);
-        const int qj = j / RPL, sj = j % RPL; // lane group and slot of row j
-        double pj = 0.0;
-        if (MODE == MODE_GE) pj = colsum<CPW>(q == qj ? reg_get(a, sj) : 0.0);
         if (n2 != 0.0) {
             // GE: w_c = scal (S - beta a_c[j]);  TS/TT: w_c = R[j][c] + scal S
-            // a_c -= tau w_c y  ==  a_c -= (tau w_c scal) x  (+ GE pivot fix-up)
+            // a_c -= tau w_c y  ==  a_c -= (tau w_c scal) x on every row but GE's row j
             const double sgp = (MODE == MODE_GE) ? fma(-beta, pj, S) : S;
             if (c > j) {
                 const double tws = (MODE == MODE_GE) ? c2 * sgp : fma(c2, S, srn * rjc);
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) a[s] = fma(-tws, xr[s], a[s]);
-                // GE: y_j = 1, not alpha * scal
-                if (MODE == MODE_GE && q == qj) reg_add(a, sj, tws * beta);
-                if (MODE != MODE_GE && q == 0) Rp[rpk(j, c)] = rjc - fma(taur, rjc, srn * S);
+                // row j is final now: R[j][c] = a_c[j] - tau w_c (GE: y_j = 1; the register
+                // copy of row j is left as is and never read again)
+                if (q == 0) Rp[rpk(j, c)] = (MODE == MODE_GE) ? pj - srn * sgp : rjc - fma(taur, rjc, srn * S);
             } else if (c == j) {
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) {
@@ -345,13 +350,18 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
                 }
                 if (q == 0) {
                     tauL[j] = taur;
-                    if (MODE != MODE_GE) dgL[j] = beta;
+                    dgL[j] = beta;
                 }
             } else if (c >= j16 && q == 0) {
                 gramL[j * 16 + (c & 15)] = scal * sgp;
             }
         }
+        if (MODE == MODE_GE && n2 == 0.0 && q == 0) { // H_j = I: row j stays as it is
+            if (c > j) Rp[rpk(j, c)] = pj;
+            else if (c == j) dgL[j] = pj;
+        }
         AVS_PANEL_MARK(3, j, c == j + 1 && q == 0 && a[0] == a[0]);
+        if (MODE == MODE_GE && j + 1 < jend) pass_row(j + 1);
         if (c == j + 1) publish(j + 1);
         AVS_PANEL_MARK(4, j, c == j + 1 && q == 0);
     }
@@ -368,15 +378,16 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
 #pragma unroll
     for (int s = 0; s < RPL; ++s) {
         const int row = r0 + s;
-        if (MODE != MODE_TT || row <= c) dst_c[(long)s * rs] = (TS)a[s];
+        // GE: the R part (above the diagonal, and the diagonal up to column B-2) comes from LDS below
+        const bool reg = (MODE == MODE_GE) ? (row > c || (row == c && c == B - 1)) : (MODE != MODE_TT || row <= c);
+        if (reg) dst_c[(long)s * rs] = (TS)a[s];
     }
-    if (MODE != MODE_GE) {
-        for (int idx = t; idx < B * B; idx += NT) {
-            int r, cc;
-            if (tr) { cc = idx % B; r = idx / B; } else { r = idx % B; cc = idx / B; }
-            if (r < cc) top[toff(r, cc, ldt, tr)] = (TS)Rp[rpk(r, cc)];
-            else if (r == cc) top[toff(r, cc, ldt, tr)] = (TS)dgL[r];
-        }
+    for (int idx = t; idx < B * B; idx += NT) {
+        int r, cc;
+        if (tr) { cc = idx % B; r = idx / B; } else { r = idx % B; cc = idx / B; }
+        if (MODE == MODE_GE && r == B - 1 && cc == B - 1) continue; // no reflector for the last column
+        if (r < cc) top[toff(r, cc, ldt, tr)] = (TS)Rp[rpk(r, cc)];
+        else if (r == cc) top[toff(r, cc, ldt, tr)] = (TS)dgL[r];
     }
 
     // diagonal blocks of the compact-WY factor, one row per lane:

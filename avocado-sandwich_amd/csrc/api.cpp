@@ -129,7 +129,7 @@ struct avs_handle {
     // per-launch timing of the stream-ordered executor (opts.reserved[1] != 0):
     // pev[i] .. pev[i+1] brackets launch i, whose kind is pkind[i]
     std::vector<hipEvent_t> pev;
-    std::vector<int> pkind;
+    std::vector<int> pkind, pmode;
     std::vector<double> pflops;
     int plaunches = 0;
     // panel stream of the stream-ordered executor and one completion event per
@@ -283,6 +283,7 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
             h->pev.push_back(e);
         }
         h->pkind.assign(nph, 0);
+        h->pmode.assign(nph, 0);
         h->pflops.assign(nph, 0.0);
     }
     if (two) {
@@ -307,6 +308,7 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
         }
         if (timing) {
             h->pkind[i] = ph.kind;
+            h->pmode[i] = ph.mode;
             h->pflops[i] = ph.kind == TK_UPDATE ? phase_update_flops(D.P, ph) : 0.0;
             HIPCHK(hipEventRecord(h->pev[2 * i], st));
         }
@@ -593,6 +595,27 @@ avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_laun
     if (update_ms) *update_ms = um;
     if (update_launches) *update_launches = un;
     if (update_flops) *update_flops = uf;
+    return check_status(h);
+}
+
+avs_status avs_phase_trace(avs_handle *h, double *start_ms, double *end_ms, int32_t *kind_mode, int64_t cap,
+                           int64_t *count)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.reserved[1] == 0 || h->plaunches == 0)
+        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.reserved[1] = 1)");
+    if (count) *count = h->plaunches;
+    if (!start_ms || !end_ms || !kind_mode) return AVS_OK;
+    if (cap < h->plaunches) return fail(-5, "cap < number of launches");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < h->plaunches; ++i) {
+        float a = 0.f, e = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, h->pev[0], h->pev[2 * i]));
+        HIPCHK(hipEventElapsedTime(&e, h->pev[0], h->pev[2 * i + 1]));
+        start_ms[i] = a;
+        end_ms[i] = e;
+        kind_mode[i] = h->pkind[i] * 4 + h->pmode[i];
+    }
     return check_status(h);
 }
 

@@ -188,6 +188,15 @@ template <int R> __device__ __forceinline__ double reg_get(const double (&a)[R],
         return v;
     }
 }
+template <int R> __device__ __forceinline__ void reg_set(double (&a)[R], int s, double d)
+{
+    if constexpr (R <= 16) {
+        a[s] = d;
+    } else {
+#pragma unroll
+        for (int u = 0; u < R; ++u) a[u] = (u == s) ? d : a[u];
+    }
+}
 template <int R> __device__ __forceinline__ void reg_add(double (&a)[R], int s, double d)
 {
     if constexpr (R <= 16) {
@@ -229,7 +238,12 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
     typedef PanelGeo<B, NT> PG;
     constexpr int CPW = PG::CPW, LPC = PG::LPC, RPL = PG::RPL, XS = PG::XS;
     static_assert(XS <= B + 128 && LPC <= 64 && CPW >= 4, "panel geometry");
-    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    // The thread index passes through an empty asm so that, where the body
+    // sits in a loop over a chain of tiles, nothing derived from it is
+    // hoisted out of the loop and held in registers across the whole body.
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int l = t & 63, w = t >> 6;
     const int c = w * CPW + (l % CPW), q = l / CPW, r0 = q * RPL;
 
     double *Rp = smem;                    // packed upper triangle of R (TS/TT); diagonal read as alpha only
@@ -272,9 +286,10 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
         double sq[RPL];
 #pragma unroll
         for (int s = 0; s < RPL; ++s) {
-            const int row = r0 + s;
-            const double v = (MODE == MODE_GE && row < p) ? 0.0 : a[s];
-            xn[xi(row)] = v;
+            // GE: the rows above p are zero in registers by now (each is zeroed when it
+            // becomes final R, which is kept in LDS)
+            const double v = a[s];
+            xn[xi(r0 + s)] = v;
             sq[s] = v * v;
         }
         nbuf[(p & 1) * LPC + q] = tree_sum(sq);
@@ -339,15 +354,14 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
                 const double tws = (MODE == MODE_GE) ? c2 * sgp : fma(c2, S, srn * rjc);
 #pragma unroll
                 for (int s = 0; s < RPL; ++s) a[s] = fma(-tws, xr[s], a[s]);
-                // row j is final now: R[j][c] = a_c[j] - tau w_c (GE: y_j = 1; the register
-                // copy of row j is left as is and never read again)
+                // row j is final now: R[j][c] = a_c[j] - tau w_c (GE: y_j = 1), kept in LDS;
+                // GE zeroes its register copy, so later publishes need no mask
                 if (q == 0) Rp[rpk(j, c)] = (MODE == MODE_GE) ? pj - srn * sgp : rjc - fma(taur, rjc, srn * S);
+                if (MODE == MODE_GE && q == j / RPL) reg_set(a, j % RPL, 0.0);
             } else if (c == j) {
+                // the stored reflector (GE: rows <= j are written from LDS at the end)
 #pragma unroll
-                for (int s = 0; s < RPL; ++s) {
-                    const int row = r0 + s;
-                    a[s] = (MODE == MODE_GE) ? (row > j ? xr[s] * scal : (row == j ? beta : a[s])) : xr[s] * scal;
-                }
+                for (int s = 0; s < RPL; ++s) a[s] = xr[s] * scal;
                 if (q == 0) {
                     tauL[j] = taur;
                     dgL[j] = beta;
@@ -356,9 +370,12 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
                 gramL[j * 16 + (c & 15)] = scal * sgp;
             }
         }
-        if (MODE == MODE_GE && n2 == 0.0 && q == 0) { // H_j = I: row j stays as it is
-            if (c > j) Rp[rpk(j, c)] = pj;
-            else if (c == j) dgL[j] = pj;
+        if (MODE == MODE_GE && n2 == 0.0) { // H_j = I: row j stays as it is
+            if (q == 0) {
+                if (c > j) Rp[rpk(j, c)] = pj;
+                else if (c == j) dgL[j] = pj;
+            }
+            if (c >= j && q == j / RPL) reg_set(a, j % RPL, 0.0);
         }
         AVS_PANEL_MARK(3, j, c == j + 1 && q == 0 && a[0] == a[0]);
         if (MODE == MODE_GE && j + 1 < jend) pass_row(j + 1);
@@ -782,7 +799,11 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
     static_assert((B * 16) % NT == 0 && PER >= 1, "update2 staging split");
     TS *YB = reinterpret_cast<TS *>(smem); // YB[buf*B*YS + k*YS + i] = Y[k][16J + i]
     TS *TL = YB + 2 * B * YS;              // TL[i*16 + jl] = T[i][16*(i/16)+jl]
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, q = lane >> 4, il = lane & 15;
+    // thread index through an empty asm: nothing derived from it is hoisted
+    // out of a caller's loop over a chain of operations
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int lane = t & 63, w = t >> 6, q = lane >> 4, il = lane & 15;
 
     // (k, i) of the u-th element this thread stages; coalesced along the
     // contiguous direction of V

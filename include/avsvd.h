@@ -130,6 +130,12 @@ avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
  * the run.  No counterpart in the reference (its benchmarks time whole calls). */
 avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_launches, double *update_ms,
                             int64_t *update_launches, double *update_flops);
+/* Per-launch timeline of the last timed stream-ordered run (opts.reserved[1]):
+ * for launch i, start and end in ms from the first launch's start event, and
+ * kind*4 + mode (kind 0 panel, 1 update; mode 0 GE, 1 TS, 2 TT).  With null
+ * arrays only *count is set.  No counterpart in the reference. */
+avs_status avs_phase_trace(avs_handle *h, double *start_ms, double *end_ms, int32_t *kind_mode, int64_t cap,
+                           int64_t *count);
 avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband);
 
 /* Full svdvals: band reduction (if not already run), band to host, block

@@ -136,6 +136,10 @@ struct avs_handle {
     // phase (the panels of a sweep run ahead of that sweep's updates)
     hipStream_t pstream = nullptr;
     std::vector<hipEvent_t> done;
+    // update stream whose CU mask leaves opts.reserved[3] CUs to the panels,
+    // and the events that join it to the handle's stream
+    hipStream_t ustream = nullptr;
+    hipEvent_t ujoin[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -295,6 +299,32 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
         }
     }
     hipStream_t us = h->stream, ps = two ? h->pstream : h->stream;
+    const int reserve = two ? h->opts.reserved[3] : 0;
+    if (reserve > 0) {
+        if (!h->ustream) {
+            int ncu = 0;
+            HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+            // at most ncu/8 - ncu/32 CUs (24 of 256): even if every cleared bit
+            // fell on one of the 8 XCDs, it keeps a quarter of its CUs
+            if (reserve > ncu / 8 - ncu / 32) return fail(AVS_ERR_UNSUPPORTED, "too many reserved CUs");
+            // the size argument counts 32-bit words; ncu words (zero past the
+            // first ncu bits) is in bounds whichever way it is read
+            std::vector<uint32_t> mask(ncu, 0);
+            for (int i = 0; i < ncu; ++i) mask[i / 32] |= 1u << (i % 32);
+            // mask bit i is a CU of XCD i % 8 (measured on MI355X): the last
+            // `reserve` bits take the same number of CUs from every XCD, which
+            // matters because workgroups are dealt to the XCDs round-robin
+            for (int r = 0; r < reserve; ++r) {
+                const int bit = ncu - 1 - r;
+                mask[bit / 32] &= ~(1u << (bit % 32));
+            }
+            HIPCHK(hipExtStreamCreateWithCUMask(&h->ustream, (uint32_t)ncu, mask.data()));
+            for (hipEvent_t &e : h->ujoin) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        HIPCHK(hipEventRecord(h->ujoin[0], h->stream));
+        HIPCHK(hipStreamWaitEvent(h->ustream, h->ujoin[0], 0));
+        us = h->ustream;
+    }
     // done[nph]: the latest update (or the matrix upload) the next sweep's panels wait for
     hipEvent_t last_update = two ? h->done[nph] : nullptr;
     if (two) HIPCHK(hipEventRecord(last_update, us));
@@ -328,8 +358,12 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
         }
         ++h->plaunches;
     }
-    // the handle's stream ends after the last panel too
-    if (two && last_panel >= 0) HIPCHK(hipStreamWaitEvent(us, h->done[last_panel], 0));
+    // the handle's stream ends after the last panel and the last update too
+    if (two && last_panel >= 0) HIPCHK(hipStreamWaitEvent(h->stream, h->done[last_panel], 0));
+    if (us != h->stream) {
+        HIPCHK(hipEventRecord(h->ujoin[1], us));
+        HIPCHK(hipStreamWaitEvent(h->stream, h->ujoin[1], 0));
+    }
     HIPCHK(hipGetLastError());
     return AVS_OK;
 }
@@ -474,6 +508,9 @@ avs_status avs_destroy(avs_handle *h)
     for (hipEvent_t e : h->pev) hipEventDestroy(e);
     for (hipEvent_t e : h->done) hipEventDestroy(e);
     if (h->pstream) hipStreamDestroy(h->pstream);
+    if (h->ustream) hipStreamDestroy(h->ustream);
+    for (hipEvent_t e : h->ujoin)
+        if (e) hipEventDestroy(e);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return AVS_OK;

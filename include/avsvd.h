@@ -81,6 +81,9 @@ typedef struct avs_opts {
                               reserved[2] != 0: AVS_SCHED_STREAMS runs panels on the
                               handle's stream too (by default they run on a second
                               stream of the handle, ahead of the same sweep's updates).
+                              reserved[3] = K > 0: AVS_SCHED_STREAMS runs the updates
+                              on a stream whose CU mask leaves K CUs (the same number
+                              on every XCD; K <= CUs/8 - CUs/32) to the panels.
                               The rest must be 0. */
 } avs_opts;
 

@@ -162,6 +162,16 @@ def bidiagonalize_band(band, n, bw):
     return d, e[: n - 1]
 
 
+def bidiagonalize_band_lapack(band, n, bw):
+    """Band -> (d, e) with LAPACK dgbbrd on one thread (the stage-2 path of the
+    batched and served forms)."""
+    band = np.asfortranarray(band, dtype=np.float64)
+    d = np.empty(n)
+    e = np.empty(max(n - 1, 1))
+    check(lib().avs_bidiagonalize_band_lapack(n, bw, dptr(band), band.shape[0], dptr(d), dptr(e)))
+    return d, e[: n - 1]
+
+
 def _dense_to_band(A, bw):
     n = A.shape[0]
     band = np.zeros((bw + 1, n), order="F")

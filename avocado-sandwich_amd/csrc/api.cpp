@@ -690,7 +690,7 @@ avs_status avs_svdvals(avs_handle *h, double *sigma)
     std::vector<double> band((size_t)(b + 1) * n), d(n), e(n > 1 ? n - 1 : 1);
     st = avs_get_band(h, band.data(), b + 1);
     if (st) return st;
-    st = avs_bidiagonalize_band(n, b, band.data(), b + 1, d.data(), e.data());
+    st = bidiagonalize_band_auto(n, b, band.data(), b + 1, d.data(), e.data());
     if (st) return st;
     return avs_bdsdc_values(n, d.data(), e.data(), sigma);
 }
@@ -1111,8 +1111,7 @@ avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
             d.resize(N);
             e.resize(std::max<long>(N - 1, 1));
             sg.resize(N);
-            avs_status s2 = bidiagonalize_band_threads(N, B->b, B->hBands + B->boff[m], B->b + 1, d.data(),
-                                                       e.data(), 1);
+            avs_status s2 = bidiagonalize_band_one(N, B->b, B->hBands + B->boff[m], B->b + 1, d.data(), e.data());
             if (!s2) s2 = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
             if (s2) {
                 std::lock_guard<std::mutex> lk(mu);

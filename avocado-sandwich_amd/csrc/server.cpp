@@ -143,9 +143,10 @@ void host_worker(avs_server *S)
         if (!r->status) {
             const int64_t N = r->N;
             std::vector<double> d(N), e(std::max<int64_t>(N - 1, 1)), sg(N);
-            const int threads = N >= 4096 ? 4 : 1;
-            avs_status st = avs::bidiagonalize_band_threads(N, r->b, r->band.data(), r->b + 1, d.data(), e.data(),
-                                                           threads);
+            // one host thread per request (dgbbrd on one thread beats the block
+            // sweeps on four at n = 4096..8192, and leaves threads to the
+            // requests that overlap)
+            avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, d.data(), e.data());
             if (!st) st = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
             if (st) {
                 r->status = st;

@@ -120,9 +120,6 @@ avs_status avs_blockbidiag(avs_handle *h, int32_t kend);
 /* tiled_QR!(A; kend). */
 avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
 
-/* Copy the band to the host in FP64 LAPACK upper-band layout:
- * band[(b + i - j) + j*ldband] = A(i, j) for max(0, j-b) <= i <= j,
- * ldband >= b+1.  Only (b+1) x n values cross PCIe, not n x n. */
 /* Per-launch device time of the last avs_blockbidiag / avs_tiled_qr run under
  * AVS_SCHED_STREAMS, for handles created with opts.reserved[1] != 0 (one event
  * pair per launch, on the stream the launch runs on; with the default second
@@ -139,6 +136,9 @@ avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_laun
  * arrays only *count is set.  No counterpart in the reference. */
 avs_status avs_phase_trace(avs_handle *h, double *start_ms, double *end_ms, int32_t *kind_mode, int64_t cap,
                            int64_t *count);
+/* Copy the band to the host in FP64 LAPACK upper-band layout:
+ * band[(b + i - j) + j*ldband] = A(i, j) for max(0, j-b) <= i <= j,
+ * ldband >= b+1.  Only (b+1) x n values cross PCIe, not n x n. */
 avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband);
 
 /* Full svdvals: band reduction (if not already run), band to host, block
@@ -160,6 +160,12 @@ const char *avs_last_error(void);
  * diagonal d[n] and super-diagonal e[n-1] of the bidiagonal result. */
 avs_status avs_bidiagonalize_band(int64_t n, int32_t bw, const double *band, int64_t ldband,
                                   double *d, double *e);
+/* The same reduction with LAPACK dgbbrd (Givens rotations) on one thread: no
+ * counterpart in the reference.  The batched and served forms use it for each
+ * matrix they give one host thread (AVSVD_STAGE2=halving selects the
+ * reference's block sweeps instead). */
+avs_status avs_bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *band, int64_t ldband,
+                                         double *d, double *e);
 /* diag_lapack, KAbasedSVD/src/bulgechasing.jl:50-53: dbdsdc('U','N').
  * sigma[n] descending.  LAPACK is loaded at run time (AVSVD_LAPACK or the
  * OpenBLAS bundled with scipy). */

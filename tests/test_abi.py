@@ -68,12 +68,31 @@ def test_host_stage_matches_oracle_bitwise(n, bw):
     np.testing.assert_array_equal(sig, O.bdsdc(d, e))
 
 
+@pytest.mark.parametrize("n,bw", [(1, 4), (5, 4), (64, 8), (300, 32), (512, 64)])
+def test_lapack_stage2_preserves_sigma(n, bw):
+    """The one-thread stage 2 of the batched and served forms (LAPACK dgbbrd):
+    sigma(d, e) = sigma(band) to FP64 rounding, and = sigma of the reference's
+    block sweeps on the same band."""
+    import scipy.linalg as sl
+    a = O.gaussian_matrix(max(n, 2), 20261211 + n + bw)[:n, :n]
+    band = np.triu(np.tril(a, bw))
+    st = avsvd.tiled._dense_to_band(band, bw)
+    d, e = avsvd.tiled.bidiagonalize_band_lapack(st, n, bw)
+    sig = avsvd.bdsdc_values(d, e)
+    ref = sl.svdvals(band) if n > 1 else np.abs(np.diag(band))
+    assert O.sigma_err(sig, ref) < 10 * max(n, 2) * np.finfo(float).eps
+    dh, eh = avsvd.bidiagonalize_band(st, n, bw)
+    assert O.sigma_err(sig, avsvd.bdsdc_values(dh, eh)) < 10 * max(n, 2) * np.finfo(float).eps
+
+
 def test_host_stage_argument_errors():
     L = lib()
     d = np.zeros(4)
     assert L.avs_bidiagonalize_band(0, 4, avsvd._lib.dptr(d), 5, avsvd._lib.dptr(d), avsvd._lib.dptr(d)) == -1
     assert L.avs_bidiagonalize_band(4, 4, avsvd._lib.dptr(d), 3, avsvd._lib.dptr(d), avsvd._lib.dptr(d)) == -4
     assert L.avs_bdsdc_values(4, avsvd._lib.dptr(d), avsvd._lib.dptr(d), None) == -4
+    assert L.avs_bidiagonalize_band_lapack(4, 4, avsvd._lib.dptr(d), 3, avsvd._lib.dptr(d),
+                                           avsvd._lib.dptr(d)) == -4
 
 
 def test_reference_api_mirror_on_host():

@@ -29,7 +29,7 @@ EXPORTS = (
     "avs_batch_create", "avs_batch_destroy", "avs_batch_set_matrix_host", "avs_batch_blockbidiag",
     "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals", "avs_set_matrix_host_sub",
     "avs_server_default_opts", "avs_server_create", "avs_server_submit", "avs_server_poll", "avs_server_wait",
-    "avs_server_stats", "avs_server_destroy",
+    "avs_server_stats", "avs_server_destroy", "avs_tune_set", "avs_tune_clear", "avs_tune_lookup",
 )
 
 
@@ -110,6 +110,9 @@ def lib():
             "avs_server_wait": [vp, ctypes.c_uint64, dp, dp],
             "avs_server_stats": [vp, dp, dp, dp, ctypes.POINTER(i64)],
             "avs_server_destroy": [vp],
+            "avs_tune_set": [i32, i64, i32],
+            "avs_tune_clear": [],
+            "avs_tune_lookup": [i32, i64, ctypes.POINTER(i32)],
         }
         for name, args in sig.items():
             f = getattr(L, name)

@@ -15,9 +15,15 @@ from ._lib import Opts, check, dptr, dtype_code, lib, vptr
 
 
 class Batch:
-    def __init__(self, ns, dtype=np.float32, b=32, workers=0):
+    """Matrices of orders `ns` reduced together; b=None takes the tile size
+    from the autotuner's table (form "batch", keyed by the largest order)."""
+
+    def __init__(self, ns, dtype=np.float32, b=None, workers=0):
         self.ns = [int(x) for x in ns]
         self.dtype = np.dtype(dtype)
+        if b is None:
+            from .autotune import choose_tile
+            b = choose_tile(max(self.ns), self.dtype, "batch")[0]
         self.b = b
         self._h = ctypes.c_void_p()
         o = Opts()
@@ -65,7 +71,7 @@ class Batch:
         return res
 
 
-def batch_svdvals(mats, dtype=np.float32, b=32):
+def batch_svdvals(mats, dtype=np.float32, b=None):
     """Singular values of every matrix in `mats` (list of square arrays)."""
     bt = Batch([m.shape[0] for m in mats], dtype=dtype, b=b)
     for i, m in enumerate(mats):

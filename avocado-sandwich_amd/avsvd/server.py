@@ -13,8 +13,24 @@ import numpy as np
 from ._lib import ServerOpts, check, dptr, dtype_code, lib, vptr
 
 
+def served_tile(dtype, n):
+    """Tile size the serving loop gives an order-n request of `dtype` whose
+    tile size is not fixed (the library's tile-size table)."""
+    b = ctypes.c_int32()
+    check(lib().avs_tune_lookup(dtype_code(np.dtype(dtype)), int(n), ctypes.byref(b)))
+    return b.value
+
+
 class Server:
-    def __init__(self, hbm_budget_gb=64, max_inflight=4, host_threads=0, b_f32=0, b_f64=0, update_mode=0):
+    """Serving loop.  b_f32 / b_f64 fix the tile size per dtype; 0 leaves it to
+    the autotuner's table (`tune_table`: a path, default the table
+    avsvd.autotune.load_table() finds), which is loaded into the library here."""
+
+    def __init__(self, hbm_budget_gb=64, max_inflight=4, host_threads=0, b_f32=0, b_f64=0, update_mode=0,
+                 tune_table=None):
+        if not (b_f32 and b_f64):
+            from .autotune import load_table, push_to_library
+            push_to_library(load_table(tune_table))
         o = ServerOpts()
         lib().avs_server_default_opts(ctypes.byref(o))
         o.hbm_budget_bytes = int(hbm_budget_gb * (1 << 30))

@@ -176,6 +176,7 @@ def _lapack_sigmas(mats):
 
 
 def suite_batched(args):
+    from .autotune import choose_tile
     from .batch import Batch
     from .tiled import TiledMatrix
     t0 = time.perf_counter()
@@ -185,7 +186,8 @@ def suite_batched(args):
     emit({"suite": "batched", "stage": "inputs", "count": len(ns), "gen_s": round(gen_s, 2),
           "sizes": {str(k): ns.count(k) for k in sorted(set(ns))}, "algorithmic_flops": flops})
     # batched FP32 form
-    bt = Batch(ns, dtype=np.float32, b=args.b)
+    bt = Batch(ns, dtype=np.float32, b=args.b or None)
+    bsel = bt.b
     for i, m in enumerate(mats):
         bt.set(i, m)
     t0 = time.perf_counter()
@@ -197,7 +199,7 @@ def suite_batched(args):
     ref = _lapack_sigmas(mats)
     lap_s = time.perf_counter() - t0
     errs = [sigma_err(s, r) for s, r in zip(sig, ref)]
-    emit({"suite": "batched", "form": "fp32 grouped launch", "count": len(ns), "b": args.b,
+    emit({"suite": "batched", "form": "fp32 grouped launch", "count": len(ns), "b": bsel,
           "band_reduction_s": round(red_ms / 1e3, 4), "band_tflops": round(flops / red_ms / 1e9, 3),
           "svdvals_total_s": round(total_s, 3), "svdvals_per_s": round(len(ns) / total_s, 2),
           "max_sigma_err_over_sigma_max": max(errs), "bar": 1e-5,
@@ -211,7 +213,7 @@ def suite_batched(args):
         if c % 16 == 0:
             progress("plain path %d/%d" % (c, len(pick)))
         A = np.asfortranarray(mats[i].astype(np.float64))
-        b = args.b if ns[i] % args.b == 0 else 32
+        b = choose_tile(ns[i], np.float64, "svdvals")[0]
         T = TiledMatrix(A, b, b)
         s = T.svdvals()
         perrs.append(sigma_err(s, ref[i]))
@@ -227,8 +229,11 @@ def suite_batched(args):
 
 def suite_serving(args):
     import torch
-    from .server import Server
+    from .autotune import load_table, push_to_library
+    from .server import Server, served_tile
     from .tiled import TiledMatrix
+    # the plain path below takes the tile size the serving loop gives each request
+    push_to_library(load_table())
     rng = np.random.default_rng(SEED + 11)
     reqs = []
     for i in range(args.requests):
@@ -257,8 +262,8 @@ def suite_serving(args):
     for c, m in enumerate(mats):
         progress("one-at-a-time %d/%d (n=%d)" % (c, len(mats), m.shape[0]))
         s0 = time.perf_counter()
-        b = 32 if m.shape[0] <= 2048 else 64
         n = m.shape[0]
+        b = served_tile(m.dtype, n)
         N = -(-n // b) * b
         A = np.zeros((N, N), dtype=m.dtype, order="F")
         A[:n, :n] = m
@@ -328,7 +333,7 @@ def main(argv=None):
     p.add_argument("--conds", type=float, nargs="+", default=[1e2, 1e4, 1e6])
     p = sub.add_parser("batched")
     p.add_argument("--count", type=int, default=4096)
-    p.add_argument("--b", type=int, default=32)
+    p.add_argument("--b", type=int, default=0, help="tile size (0: the autotuner's table)")
     p.add_argument("--plain-sample", type=int, default=128)
     p = sub.add_parser("serving")
     p.add_argument("--requests", type=int, default=48)

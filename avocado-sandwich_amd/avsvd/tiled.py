@@ -212,9 +212,15 @@ def my_CU_svdval(A, block_size):
     return diag_lapack(bidiagonalize(A, block_size))
 
 
-def svdvals(A, tilesize=64, **opts):
-    """Singular values of a square host matrix through the whole path."""
+def svdvals(A, tilesize=None, **opts):
+    """Singular values of a square host matrix through the whole path.  Without
+    a tile size the autotuner's table picks it (form "svdvals")."""
     A = np.asfortranarray(A)
+    if tilesize is None:
+        from .autotune import choose_tile
+        tilesize, dom = choose_tile(A.shape[0], A.dtype, "svdvals")
+        if dom and "domain" not in opts:
+            opts["domain"] = dom
     return TiledMatrix(A, tilesize, tilesize, **opts).svdvals()
 
 

@@ -49,12 +49,37 @@ struct Request {
     bool done = false;
 };
 
+// Tile sizes chosen by the autotuner: for each dtype, entries (n_upto, b)
+// sorted by n_upto; a request of order n takes the first entry with
+// n <= n_upto, or the last entry when n is larger than every n_upto.
+struct TuneEntry {
+    int64_t n_upto;
+    int32_t b;
+};
+std::mutex g_tune_mu;
+std::map<int, std::vector<TuneEntry>> g_tune;
+
+int default_b(int64_t n)
+{
+    // the host stage cost grows with b; small tiles keep it cheap
+    return n <= 2048 ? 32 : 64;
+}
+
+int tuned_b(int dtype, int64_t n)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(dtype);
+    if (it == g_tune.end() || it->second.empty()) return default_b(n);
+    for (const TuneEntry &e : it->second)
+        if (n <= e.n_upto) return e.b;
+    return it->second.back().b;
+}
+
 int pick_b(const avs_server_opts &o, int dtype, int64_t n)
 {
     const int want = dtype == AVS_F64 ? o.b_f64 : o.b_f32;
     if (want == 32 || want == 64 || want == 128) return want;
-    // the host stage cost grows with b; small tiles keep it cheap
-    return n <= 2048 ? 32 : 64;
+    return tuned_b(dtype, n);
 }
 
 size_t footprint_of(int dtype, int64_t N, int b)
@@ -173,6 +198,36 @@ void host_worker(avs_server *S)
 } // namespace
 
 extern "C" {
+
+avs_status avs_tune_set(int32_t dtype, int64_t n_upto, int32_t b)
+{
+    if (dtype != AVS_F32 && dtype != AVS_F64) return fail(-1, "dtype must be AVS_F32 or AVS_F64");
+    if (n_upto < 1) return fail(-2, "n_upto must be positive");
+    if (b != 32 && b != 64 && b != 128) return fail(-3, "tile size must be 32, 64 or 128");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::vector<TuneEntry> &v = g_tune[dtype];
+    auto it = std::find_if(v.begin(), v.end(), [&](const TuneEntry &e) { return e.n_upto == n_upto; });
+    if (it != v.end()) it->b = b;
+    else v.push_back({n_upto, b});
+    std::sort(v.begin(), v.end(), [](const TuneEntry &x, const TuneEntry &y) { return x.n_upto < y.n_upto; });
+    return AVS_OK;
+}
+
+avs_status avs_tune_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+    return AVS_OK;
+}
+
+avs_status avs_tune_lookup(int32_t dtype, int64_t n, int32_t *b)
+{
+    if (dtype != AVS_F32 && dtype != AVS_F64) return fail(-1, "dtype must be AVS_F32 or AVS_F64");
+    if (n < 1) return fail(-2, "n must be positive");
+    if (!b) return fail(-3, "null b");
+    *b = tuned_b(dtype, n);
+    return AVS_OK;
+}
 
 void avs_server_default_opts(avs_server_opts *o)
 {

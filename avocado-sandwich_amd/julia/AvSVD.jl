@@ -17,7 +17,7 @@
 module AvSVD
 
 export TiledMatrix, tiled_QR!, Blockbidiag!, OOC_QR!, OOC_Bidiag!, OOC_SVD!, compile_kernels,
-       bidiagonalize, my_CU_svdval, diag_lapack, svdvals_gpu
+       bidiagonalize, my_CU_svdval, diag_lapack, svdvals_gpu, tune_set!
 
 const libavsvd = get(ENV, "AVSVD_LIB", joinpath(@__DIR__, "..", "libavsvd.so"))
 
@@ -58,6 +58,11 @@ function compile_kernels(; tilesize::Int=32, tilefactor::Int=1, tiledim2d::Bool=
     TILE[] = tilesize
     return nothing
 end
+
+# tune_set!(T, n_upto, b): one entry of the library's tile-size table (the
+# autotuner's measured choice for served requests of order <= n_upto).
+tune_set!(::Type{T}, n_upto::Integer, b::Integer) where {T<:Union{Float32,Float64}} =
+    check(ccall((:avs_tune_set, libavsvd), Int32, (Int32, Int64, Int32), dtype(T), n_upto, b))
 
 mutable struct TiledMatrix{T<:Union{Float32,Float64}}
     handle::Ptr{Cvoid}

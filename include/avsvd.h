@@ -236,7 +236,7 @@ typedef struct avs_server_opts {
     int64_t hbm_budget_bytes; /* device working-set budget for admitted requests */
     int32_t max_inflight;     /* concurrent device reductions (streams) */
     int32_t host_threads;     /* host-stage workers */
-    int32_t b_f32, b_f64;     /* tile sizes (0 = 32 up to n=2048, else 64) */
+    int32_t b_f32, b_f64;     /* tile sizes (0 = the tile-size table below) */
     int32_t update_mode;      /* avs_update_mode for FP32 requests */
     int32_t reserved[7];
 } avs_server_opts;
@@ -251,6 +251,21 @@ avs_status avs_server_wait(avs_server *server, uint64_t req_id, double *sigma, d
 avs_status avs_server_stats(avs_server *server, double *p50_ms, double *p99_ms, double *solves_per_s,
                             int64_t *completed);
 avs_status avs_server_destroy(avs_server *server);
+
+/* ---- tile-size table (filled by the autotuner) ----
+ * Per dtype, entries (n_upto, b): a served request of order n whose tile size
+ * is not fixed by avs_server_opts takes the b of the first entry with
+ * n <= n_upto, or of the last entry when n exceeds them all; with no entries
+ * for its dtype, b = 32 up to n = 2048, else 64.  avs_tune_set replaces the
+ * entry with the same n_upto.  The table is process-wide and thread-safe.  No
+ * reference counterpart: the reference fixes its tile size by hand before
+ * compiling its kernels (KAbasedSVD/src/QRkernelscompile.jl:1-4,
+ * tiledalgos.jl:262-267); here every tile size is built ahead of time and the
+ * choice is a measured table.  Errors: -1 dtype, -2 n_upto or n < 1, -3 tile
+ * size not in {32, 64, 128} (avs_tune_set) or null b (avs_tune_lookup). */
+avs_status avs_tune_set(int32_t dtype, int64_t n_upto, int32_t b);
+avs_status avs_tune_clear(void);
+avs_status avs_tune_lookup(int32_t dtype, int64_t n, int32_t *b);
 
 /* ---- diagnostics ----
  * avs_debug_mfma_layout: runs one v_mfma_f64_16x16x4f64 and one

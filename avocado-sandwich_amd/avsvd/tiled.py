@@ -172,6 +172,16 @@ def bidiagonalize_band_lapack(band, n, bw):
     return d, e[: n - 1]
 
 
+def bidiagonalize_band_chase(band, n, bw):
+    """Band -> (d, e) by Householder bulge chasing, the sweeps pipelined over
+    the library's host threads (the default stage 2)."""
+    band = np.asfortranarray(band, dtype=np.float64)
+    d = np.empty(n)
+    e = np.empty(max(n - 1, 1))
+    check(lib().avs_bidiagonalize_band_chase(n, bw, dptr(band), band.shape[0], dptr(d), dptr(e)))
+    return d, e[: n - 1]
+
+
 def _dense_to_band(A, bw):
     n = A.shape[0]
     band = np.zeros((bw + 1, n), order="F")

@@ -1111,7 +1111,7 @@ avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
             d.resize(N);
             e.resize(std::max<long>(N - 1, 1));
             sg.resize(N);
-            avs_status s2 = bidiagonalize_band_one(N, B->b, B->hBands + B->boff[m], B->b + 1, d.data(), e.data());
+            avs_status s2 = bidiagonalize_band_one(N, B->b, B->hBands + B->boff[m], B->b + 1, d.data(), e.data(), 1);
             if (!s2) s2 = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
             if (s2) {
                 std::lock_guard<std::mutex> lk(mu);

@@ -271,6 +271,12 @@ avs_status avs_bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *ba
     return avs::bidiagonalize_band_lapack(n, bw, band, ldband, d, e);
 }
 
+avs_status avs_bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d,
+                                        double *e)
+{
+    return avs::bidiagonalize_band_chase(n, bw, band, ldband, d, e, host_threads());
+}
+
 } // extern "C"
 
 namespace avs {
@@ -305,10 +311,7 @@ avs_status bidiagonalize_band_threads(int64_t n, int32_t bw, const double *band,
 int host_thread_count() { return host_threads(); }
 
 // Band -> bidiagonal on one thread with LAPACK dgbbrd (Givens rotations, the
-// band chased in place).  Used where each matrix gets one host thread (the
-// batched and served forms): on one thread it is 2.8-4.6x faster than the
-// bandwidth-halving block sweeps at n = 2048..4096 (which win only when their
-// sweeps are pipelined over many threads on one matrix).  Our band layout
+// band chased in place); selected by AVSVD_STAGE2=dgbbrd.  Our band layout
 // (band[(bw + i - j) + j*ldband] = A(i, j)) is LAPACK's band storage with
 // kl = 0, ku = bw.
 avs_status bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e)
@@ -337,29 +340,31 @@ avs_status bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *band, 
     return AVS_OK;
 }
 
-// The stage-2 reduction for a caller that gives each matrix one host thread:
-// dgbbrd when LAPACK provides it (unless AVSVD_STAGE2=halving), else the
-// bandwidth-halving sweeps on one thread.
-avs_status bidiagonalize_band_one(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e)
+// The stage-2 reduction for a caller that gives each matrix `nthreads` host
+// threads (the batched form: 1; the serving loop: 1, or 4 for n >= 4096):
+// Householder bulge chasing (bandbrd.cpp), 1.5-2.8x faster than dgbbrd on
+// one thread at n = 256..4096, bw = 32..64 and 4x faster than the
+// bandwidth-halving sweeps on 8 threads at n = 8192.  AVSVD_STAGE2=dgbbrd or
+// =halving selects LAPACK dgbbrd or the reference's halving sweeps instead.
+avs_status bidiagonalize_band_one(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,
+                                  int nthreads)
 {
     const char *pol = getenv("AVSVD_STAGE2");
-    const bool halving = pol && strcmp(pol, "halving") == 0;
-    if (!halving) {
+    if (pol && strcmp(pol, "halving") == 0) return bidiagonalize_band_threads(n, bw, band, ldband, d, e, nthreads);
+    if (pol && strcmp(pol, "dgbbrd") == 0) {
         std::call_once(g_lapack_once, load_lapack);
         if (g_dgbbrd) return bidiagonalize_band_lapack(n, bw, band, ldband, d, e);
     }
-    return bidiagonalize_band_threads(n, bw, band, ldband, d, e, 1);
+    return bidiagonalize_band_chase(n, bw, band, ldband, d, e, nthreads);
 }
 
 // The stage-2 reduction for a caller that has the host threads to itself (the
-// plain per-matrix svdvals): below n*bw = 32768 one thread of dgbbrd beats the
-// sweeps pipelined over 8 threads (0.48 vs 1.3 ms at n=64, 57 vs 65 ms at
-// n=1024, bw=32); above it the pipelined sweeps win (421 vs 494 ms at n=2048,
-// bw=64).
+// plain per-matrix svdvals): the bulge chasing on one thread below
+// n*bw = 65536 (threads do not pay off there), pipelined over every host
+// thread above.
 avs_status bidiagonalize_band_auto(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e)
 {
-    if (n * (int64_t)bw <= 32768) return bidiagonalize_band_one(n, bw, band, ldband, d, e);
-    return bidiagonalize_band_threads(n, bw, band, ldband, d, e, host_threads());
+    return bidiagonalize_band_one(n, bw, band, ldband, d, e, n * (int64_t)bw <= 65536 ? 1 : host_threads());
 }
 
 } // namespace avs

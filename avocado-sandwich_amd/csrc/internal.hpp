@@ -11,8 +11,13 @@ avs_status bidiagonalize_band_threads(int64_t n, int32_t bw, const double *band,
 // stage 2 with LAPACK dgbbrd on one thread
 avs_status bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e);
 // stage 2 for a caller that gives each matrix one host thread (dgbbrd if available)
-avs_status bidiagonalize_band_one(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e);
+avs_status bidiagonalize_band_one(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,
+                                  int nthreads);
 avs_status bidiagonalize_band_auto(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e);
+// band -> bidiagonal by Householder bulge chasing, sweeps pipelined over
+// nthreads host threads (bandbrd.cpp); the result does not depend on nthreads
+avs_status bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,
+                                    int nthreads);
 // host threads the library uses (AVSVD_HOST_THREADS, else min(16, cores))
 int host_thread_count();
 } // namespace avs

@@ -168,10 +168,10 @@ void host_worker(avs_server *S)
         if (!r->status) {
             const int64_t N = r->N;
             std::vector<double> d(N), e(std::max<int64_t>(N - 1, 1)), sg(N);
-            // one host thread per request (dgbbrd on one thread beats the block
-            // sweeps on four at n = 4096..8192, and leaves threads to the
-            // requests that overlap)
-            avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, d.data(), e.data());
+            // one host thread per request, four from n = 4096 (the largest
+            // requests set the p99; the sweeps pipeline well there)
+            avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, d.data(), e.data(),
+                                                        N >= 4096 ? 4 : 1);
             if (!st) st = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
             if (st) {
                 r->status = st;

@@ -166,6 +166,17 @@ avs_status avs_bidiagonalize_band(int64_t n, int32_t bw, const double *band, int
  * reference's block sweeps instead). */
 avs_status avs_bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *band, int64_t ldband,
                                          double *d, double *e);
+/* avs_bidiagonalize_band_chase: the same reduction by Householder bulge
+ * chasing (one length-bw reflector per annihilated row or column segment,
+ * Lang's order) on a compact band store with unit-stride inner loops, the
+ * sweeps pipelined over the library's host threads (AVSVD_HOST_THREADS,
+ * default min(16, cores)); the result does not depend on the thread count.
+ * No reference counterpart (the reference halves the bandwidth with block
+ * sweeps, KAbasedSVD/src/bulgechasing.jl:33-42); singular values are
+ * unchanged, d and e may differ in sign from the other two routines.
+ * Argument errors as avs_bidiagonalize_band. */
+avs_status avs_bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d,
+                                        double *e);
 /* diag_lapack, KAbasedSVD/src/bulgechasing.jl:50-53: dbdsdc('U','N').
  * sigma[n] descending.  LAPACK is loaded at run time (AVSVD_LAPACK or the
  * OpenBLAS bundled with scipy). */

@@ -85,6 +85,52 @@ def test_lapack_stage2_preserves_sigma(n, bw):
     assert O.sigma_err(sig, avsvd.bdsdc_values(dh, eh)) < 10 * max(n, 2) * np.finfo(float).eps
 
 
+@pytest.mark.parametrize("n,bw", [(1, 4), (2, 4), (5, 4), (7, 3), (64, 8), (100, 1), (257, 31), (300, 32),
+                                  (512, 64), (700, 16)])
+def test_chase_stage2_preserves_sigma(n, bw):
+    """The Householder bulge chasing (default stage 2 of the batched, served and
+    plain forms): sigma(d, e) = sigma(band) to FP64 rounding, the same as
+    LAPACK dgbbrd's, and bit-identical for 1 and 4 pipelined host threads."""
+    import scipy.linalg as sl
+    a = O.gaussian_matrix(max(n, 2), 20261231 + n + bw)[:n, :n]
+    band = np.triu(np.tril(a, bw))
+    st = avsvd.tiled._dense_to_band(band, bw)
+    outs = []
+    old = os.environ.get("AVSVD_HOST_THREADS")
+    try:
+        for th in ("1", "4"):
+            os.environ["AVSVD_HOST_THREADS"] = th
+            outs.append(avsvd.tiled.bidiagonalize_band_chase(st, n, bw))
+    finally:
+        if old is None:
+            os.environ.pop("AVSVD_HOST_THREADS", None)
+        else:
+            os.environ["AVSVD_HOST_THREADS"] = old
+    (d, e), (d4, e4) = outs
+    np.testing.assert_array_equal(d, d4)
+    np.testing.assert_array_equal(e, e4)
+    sig = avsvd.bdsdc_values(d, e)
+    ref = sl.svdvals(band) if n > 1 else np.abs(np.diag(band))
+    assert O.sigma_err(sig, ref) < 10 * max(n, 2) * np.finfo(float).eps
+    dl, el = avsvd.tiled.bidiagonalize_band_lapack(st, n, bw)
+    assert O.sigma_err(sig, avsvd.bdsdc_values(dl, el)) < 10 * max(n, 2) * np.finfo(float).eps
+
+
+def test_chase_stage2_graded_band():
+    """A band whose entries span 1e-300 .. 1: the chased bulges decay to
+    subnormal sizes without producing Inf or NaN (divisions, not reciprocals)."""
+    import scipy.linalg as sl
+    n, bw = 600, 16
+    a = O.gaussian_matrix(n, 20261237)
+    g = np.logspace(0, -300, n)
+    band = np.triu(np.tril(a * g[:, None], bw))
+    st = avsvd.tiled._dense_to_band(band, bw)
+    d, e = avsvd.tiled.bidiagonalize_band_chase(st, n, bw)
+    assert np.isfinite(d).all() and np.isfinite(e).all()
+    sig = avsvd.bdsdc_values(d, e)
+    assert O.sigma_err(sig, sl.svdvals(band)) < 10 * n * np.finfo(float).eps
+
+
 def test_host_stage_argument_errors():
     L = lib()
     d = np.zeros(4)
@@ -93,6 +139,11 @@ def test_host_stage_argument_errors():
     assert L.avs_bdsdc_values(4, avsvd._lib.dptr(d), avsvd._lib.dptr(d), None) == -4
     assert L.avs_bidiagonalize_band_lapack(4, 4, avsvd._lib.dptr(d), 3, avsvd._lib.dptr(d),
                                            avsvd._lib.dptr(d)) == -4
+    assert L.avs_bidiagonalize_band_chase(4, 4, avsvd._lib.dptr(d), 3, avsvd._lib.dptr(d),
+                                          avsvd._lib.dptr(d)) == -4
+    assert L.avs_bidiagonalize_band_chase(4, 0, avsvd._lib.dptr(d), 5, avsvd._lib.dptr(d),
+                                          avsvd._lib.dptr(d)) == -2
+    assert L.avs_bidiagonalize_band_chase(4, 4, avsvd._lib.dptr(d), 5, avsvd._lib.dptr(d), None) == -6
 
 
 def test_reference_api_mirror_on_host():

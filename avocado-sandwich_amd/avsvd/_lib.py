@@ -30,6 +30,7 @@ EXPORTS = (
     "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals", "avs_set_matrix_host_sub",
     "avs_server_default_opts", "avs_server_create", "avs_server_submit", "avs_server_poll", "avs_server_wait",
     "avs_server_stats", "avs_server_destroy", "avs_tune_set", "avs_tune_clear", "avs_tune_lookup",
+    "avs_device_arch",
 )
 
 
@@ -59,6 +60,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise OSError("libavsvd.so not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        # torch ships its own HIP runtime; once another one is loaded first,
+        # torch's device initialisation breaks the process's HIP state.  Load
+        # torch (when installed) before the library so both use torch's.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
         L.avs_last_error.restype = ctypes.c_char_p
@@ -114,6 +122,7 @@ def lib():
             "avs_tune_set": [i32, i64, i32],
             "avs_tune_clear": [],
             "avs_tune_lookup": [i32, i64, ctypes.POINTER(i32)],
+            "avs_device_arch": [i32, ctypes.c_char_p, i64],
         }
         for name, args in sig.items():
             f = getattr(L, name)

@@ -23,6 +23,7 @@ with the package for the device's architecture (tune_<arch>.json).
     python -m avsvd.autotune batch --count 4096 --out tune.json
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -46,14 +47,13 @@ def bucket(n):
 
 
 def device_arch():
-    """gfx architecture of device 0 (e.g. 'gfx950'), or None without a GPU."""
-    try:
-        import torch
-        if not torch.cuda.is_available():
-            return None
-        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
-    except Exception:
+    """gfx architecture of device 0 (e.g. 'gfx950'), or None without a GPU.
+    Asked of the library's HIP runtime, so that a process which has not
+    imported torch does not load a second HIP runtime here."""
+    buf = ctypes.create_string_buffer(64)
+    if lib().avs_device_arch(0, buf, len(buf)) != 0:
         return None
+    return buf.value.decode() or None
 
 
 def _dtname(dtype):

@@ -600,6 +600,21 @@ avs_status avs_synchronize(avs_handle *h)
     return check_status(h);
 }
 
+avs_status avs_device_arch(int32_t device, char *name, int64_t len)
+{
+    if (!name) return fail(-2, "null name");
+    if (len < 2) return fail(-3, "len < 2");
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    int64_t k = 0;
+    while (k + 1 < len && p.gcnArchName[k] && p.gcnArchName[k] != ':') {
+        name[k] = p.gcnArchName[k];
+        ++k;
+    }
+    name[k] = 0;
+    return AVS_OK;
+}
+
 avs_status avs_last_time_ms(avs_handle *h, double *ms)
 {
     if (!h) return fail(-1, "null handle");

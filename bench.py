@@ -63,8 +63,8 @@ def main():
                          "hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only (this engine runs one matrix per GPU; see the sharding design "
-              "in DESIGN.md); got --gpus %d" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only (the multi-GPU sharding of the reduction is a design, "
+              "not built); got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import ctypes

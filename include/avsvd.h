@@ -48,7 +48,7 @@ typedef enum { AVS_F64 = 0, AVS_F32 = 1 } avs_dtype;
 /* How the tile DAG is executed. */
 typedef enum {
     AVS_SCHED_PERSISTENT = 0, /* one device-side dependency-counting persistent launch (disabled:
-                                 returns AVS_ERR_UNSUPPORTED) */
+                                 returns AVS_ERR_UNSUPPORTED; its launch did not complete) */
     AVS_SCHED_HOST = 1,       /* host-enqueued launches in the reference order (plain path) */
     AVS_SCHED_STREAMS = 2     /* fused per-domain chain kernels, stream-ordered */
 } avs_sched;
@@ -152,6 +152,11 @@ avs_status avs_synchronize(avs_handle *h);
 avs_status avs_last_time_ms(avs_handle *h, double *ms);
 
 const char *avs_last_error(void);
+/* gfx architecture of HIP device `device` without target features (e.g.
+ * "gfx950"), NUL-terminated in name[len] (the autotuner's table key).  No
+ * reference counterpart.  Errors: -2 null name, -3 len < 2, AVS_ERR_HIP when
+ * the device cannot be queried. */
+avs_status avs_device_arch(int32_t device, char *name, int64_t len);
 
 /* ---- host stage (kept on the host in FP64, as the reference runs it) ---- */
 

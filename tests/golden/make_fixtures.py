@@ -68,7 +68,7 @@ def main():
     for n in (16, 24, 64, 96, 128, 256, 512, 1024):
         a = oracle.gaussian_matrix(n, BASE + n)
         sig["gauss_n%d" % n] = sl.svd(a, compute_uv=False, lapack_driver="gesdd")
-    # n=1024, seed index 1
+    # the CPU plain-path workload: n=1024, seed index 1
     sig["config1_n1024"] = sl.svd(oracle.gaussian_matrix(1024, BASE + 1), compute_uv=False,
                                   lapack_driver="gesdd")
     np.savez_compressed(os.path.join(HERE, "sigma_small.npz"), **sig)

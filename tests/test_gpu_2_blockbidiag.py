@@ -48,7 +48,7 @@ CASES = [
 ]
 
 PERSISTENT_OFF = ("persistent dependency-counting executor disabled: its launch did not complete on the "
-                  "device (twice) and the cause is not identified; see DESIGN.md")
+                  "device (twice) and the cause is not identified")
 
 
 @pytest.mark.skip(reason=PERSISTENT_OFF)
@@ -161,7 +161,7 @@ def test_kend_partial_matches_oracle(gpu, sched):
 
 
 def test_config1_svdvals_end_to_end(gpu, golden_dir):
-    """BASELINE config 1 on the device path: n=1024, b=64, FP64."""
+    """The CPU plain-path workload (n=1024, b=64, FP64, seed index 1) on the device path."""
     n, b = 1024, 64
     a = np.asfortranarray(O.gaussian_matrix(n, BASE + 1))
     ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["config1_n1024"]

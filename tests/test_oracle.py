@@ -142,7 +142,7 @@ def test_bidiagonalize_preserves_sigma(n, bw):
 
 
 def test_config1_pipeline_against_gesdd(golden_dir):
-    # n=1024, b=64 Gaussian through band + host stage
+    # the CPU plain-path workload: n=1024, b=64 Gaussian through band + host stage
     n, b = 1024, 64
     a = O.gaussian_matrix(n, BASE + 1)
     ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["config1_n1024"]

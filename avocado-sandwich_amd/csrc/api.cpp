@@ -909,6 +909,10 @@ struct avs_batch {
     bool reduced = false, timed = false;
     double *hBands = nullptr; // pinned host copy of every band (band_elems doubles)
     bool bands_valid = false;
+    // device stage 2: band stores ((3b+1) x N per matrix), bidiagonals (2N per
+    // matrix), out-of-store flags, pinned host copy of the bidiagonals
+    double *dChase = nullptr, *dDE = nullptr, *hDE = nullptr;
+    int *dChaseErr = nullptr;
 };
 
 extern "C" {
@@ -1000,12 +1004,13 @@ avs_status avs_batch_destroy(avs_batch *B)
 {
     if (!B) return AVS_OK;
     if (B->stream) hipStreamSynchronize(B->stream);
-    void *ptrs[] = {B->dArena, B->dStage, B->dBands, B->dScratch, B->dOff, B->dSoff, B->dTile0,
+    void *ptrs[] = {B->dChase, B->dDE, B->dChaseErr, B->dArena, B->dStage, B->dBands, B->dScratch, B->dOff, B->dSoff, B->dTile0,
                     B->dCol0, B->dBoff, B->dNts, B->dOrder, B->dNext};
     for (void *p : ptrs) hipFree(p);
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
     if (B->hBands) hipHostFree(B->hBands);
+    if (B->hDE) hipHostFree(B->hDE);
     if (B->own_stream) hipStreamDestroy(B->stream);
     delete B;
     return AVS_OK;
@@ -1097,6 +1102,46 @@ avs_status avs_batch_get_band(avs_batch *B, int32_t idx, double *band, int64_t l
     return AVS_OK;
 }
 
+// Stage 2 on the device: every matrix's band -> bidiagonal in one launch (one
+// wavefront per matrix), then only (d, e) comes back.
+static avs_status batch_device_chase(avs_batch *B)
+{
+    const size_t cols = (size_t)B->total_cols;
+    if (!B->dChase) {
+        // 32 spare columns: the device kernel's unrolled loads run up to 31
+        // strides past a matrix's last column (their values are not used)
+        HIPCHK(hipMalloc(&B->dChase, sizeof(double) * (cols + 32) * (3 * (size_t)B->b + 1)));
+        HIPCHK(hipMalloc(&B->dDE, sizeof(double) * 2 * cols));
+        HIPCHK(hipMalloc(&B->dChaseErr, sizeof(int) * (size_t)B->count));
+        HIPCHK(hipHostMalloc((void **)&B->hDE, sizeof(double) * 2 * cols, hipHostMallocDefault));
+    }
+    HIPCHK(hipMemsetAsync(B->dChaseErr, 0, sizeof(int) * (size_t)B->count, B->stream));
+    avs_status st = dispatch_tb(B->dtype, B->b, [&](auto tp, auto) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        launch_batch_chase<TS>(B->dArena, B->dOff, B->dNts, B->dOrder, B->dCol0, B->count, B->b, B->dChase, B->dDE,
+                               B->dChaseErr, B->stream);
+        HIPCHK(hipGetLastError());
+        return AVS_OK;
+    });
+    if (st) return st;
+    std::vector<int> errs(B->count);
+    HIPCHK(hipMemcpyAsync(B->hDE, B->dDE, sizeof(double) * 2 * cols, hipMemcpyDeviceToHost, B->stream));
+    HIPCHK(hipMemcpyAsync(errs.data(), B->dChaseErr, sizeof(int) * (size_t)B->count, hipMemcpyDeviceToHost,
+                          B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    for (int m = 0; m < B->count; ++m)
+        if (errs[m]) return fail(AVS_ERR_STATE, "device band chase left its band store (matrix %d)", m);
+    return AVS_OK;
+}
+
+// stage 2 of the batch on the device unless AVSVD_STAGE2 asks for a host
+// routine (host, dgbbrd, halving); the device kernel handles b <= 32
+static bool batch_stage2_on_device(const avs_batch *B)
+{
+    const char *pol = getenv("AVSVD_STAGE2");
+    return B->b <= 32 && !(pol && *pol);
+}
+
 avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
 {
     if (!B) return fail(-1, "null batch");
@@ -1106,7 +1151,8 @@ avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
         if ((st = avs_batch_blockbidiag(B))) return st;
     }
     B->bands_valid = false;
-    if ((st = batch_fetch_bands(B))) return st;
+    const bool dev2 = batch_stage2_on_device(B);
+    if ((st = dev2 ? batch_device_chase(B) : batch_fetch_bands(B))) return st;
     std::vector<long> soff(B->count, 0);
     for (int i = 1; i < B->count; ++i) soff[i] = soff[i - 1] + B->n[i - 1];
     std::vector<int> order(B->count);
@@ -1126,7 +1172,14 @@ avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
             d.resize(N);
             e.resize(std::max<long>(N - 1, 1));
             sg.resize(N);
-            avs_status s2 = bidiagonalize_band_one(N, B->b, B->hBands + B->boff[m], B->b + 1, d.data(), e.data(), 1);
+            avs_status s2 = AVS_OK;
+            if (dev2) {
+                const double *de = B->hDE + 2 * B->col0[m];
+                memcpy(d.data(), de, sizeof(double) * N);
+                if (N > 1) memcpy(e.data(), de + N, sizeof(double) * (N - 1));
+            } else {
+                s2 = bidiagonalize_band_one(N, B->b, B->hBands + B->boff[m], B->b + 1, d.data(), e.data(), 1);
+            }
             if (!s2) s2 = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
             if (s2) {
                 std::lock_guard<std::mutex> lk(mu);

@@ -39,6 +39,11 @@ template <typename TS>
 void launch_batch_band(const void *arena, const long *off, const int *nts, const long *col0, long total_cols,
                        double *bands, const long *boff, int count, int b, hipStream_t s);
 
+// band -> bidiagonal on the device, one wavefront per matrix (chase.hip); b <= 32
+template <typename TS>
+void launch_batch_chase(const void *arena, const long *off, const int *nts, const int *order, const long *col0,
+                        int count, int b, double *store, double *de, int *err, hipStream_t s);
+
 template <typename TS, int B>
 void launch_chain_panel(int nchains, int mode, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
                         const int *chain_start, int chain0, hipStream_t s);

@@ -211,6 +211,32 @@ def test_batched_mixed_sizes_with_padding(gpu, dt):
     np.testing.assert_array_equal(bt.band(2), T.band())
 
 
+def test_batched_device_stage2_matches_host(gpu, monkeypatch):
+    """The batch's band -> bidiagonal on the device (one wavefront per matrix,
+    the default for b <= 32) gives the sigma of the host bulge chasing of the
+    same bands, to 10*n*eps of FP64, and both match FP64 LAPACK to the
+    working precision's bar."""
+    rng = np.random.default_rng(17)
+    ns = [1, 2, 31, 32, 33, 64, 97, 200, 256, 300]
+    mats = [(rng.standard_normal((n, n)) / np.sqrt(n)) for n in ns]
+    outs = []
+    for pol in ("", "host"):
+        print("case batched stage 2 policy=%r" % (pol or "device"), flush=True)
+        if pol:
+            monkeypatch.setenv("AVSVD_STAGE2", pol)
+        else:
+            monkeypatch.delenv("AVSVD_STAGE2", raising=False)
+        bt = avsvd.Batch(ns, dtype=np.float64, b=32)
+        for i, m in enumerate(mats):
+            bt.set(i, m)
+        outs.append(bt.svdvals())
+        del bt
+    for n, m, dv, hs in zip(ns, mats, outs[0], outs[1]):
+        bar = 10 * max(n, 2) * np.finfo(np.float64).eps
+        assert O.sigma_err(dv, hs) <= bar, n
+        assert O.sigma_err(dv, sl.svdvals(m)) <= bar, n
+
+
 def test_serving_results_same_alone_and_in_mix(gpu):
     """Continuous batching: each request's sigma is the same when it is served
     alone and when it overlaps with other requests on the device (the task DAG

@@ -84,6 +84,11 @@ typedef struct avs_opts {
                               reserved[3] = K > 0: AVS_SCHED_STREAMS runs the updates
                               on a stream whose CU mask leaves K CUs (the same number
                               on every XCD; K <= CUs/8 - CUs/32) to the panels.
+                              reserved[4] != 0: AVS_SCHED_STREAMS captures the
+                              first run of a handle's phase program as a HIP
+                              graph and replays it on every run (not with
+                              reserved[1..3]); by default every launch is
+                              enqueued each run (faster on MI355X).
                               The rest must be 0. */
 } avs_opts;
 

@@ -211,6 +211,26 @@ def test_batched_mixed_sizes_with_padding(gpu, dt):
     np.testing.assert_array_equal(bt.band(2), T.band())
 
 
+@pytest.mark.parametrize("n,b,dt", [(1024, 128, np.float64), (512, 32, np.float32)])
+def test_graph_replays_match_eager(gpu, n, b, dt):
+    """The stream-ordered run replayed from its captured HIP graph gives the
+    eager run's band bit for bit, on every replay, with eager uploads and
+    synchronisations between the replays."""
+    a = O.gaussian_matrix(n, BASE + 7 * n + b).astype(dt)
+    E = avsvd.TiledMatrix(np.asfortranarray(a.copy()), b, b)
+    avsvd.Blockbidiag_(E, sync=False)
+    ref = E.band()
+    del E
+    T = avsvd.TiledMatrix(np.asfortranarray(a.copy()), b, b, graph=True)
+    for rep in range(4):  # capture + replay, then replays
+        print("case graph n=%d b=%d rep=%d" % (n, b, rep), flush=True)
+        T.A[:] = a
+        T.reload()
+        avsvd.Blockbidiag_(T, sync=False)
+        np.testing.assert_array_equal(T.band(), ref)
+        T.synchronize()
+
+
 def test_batched_device_stage2_matches_host(gpu, monkeypatch):
     """The batch's band -> bidiagonal on the device (one wavefront per matrix,
     the default for b <= 32) gives the sigma of the host bulge chasing of the

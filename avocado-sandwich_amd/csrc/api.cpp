@@ -964,6 +964,11 @@ struct avs_batch {
     // matrix), out-of-store flags, pinned host copy of the bidiagonals
     double *dChase = nullptr, *dDE = nullptr, *hDE = nullptr;
     int *dChaseErr = nullptr;
+    // device singular values: workgroup table (matrix, first value), true
+    // sizes and output offsets per matrix, the values
+    int *dWmat = nullptr, *dWfirst = nullptr, nbis = 0;
+    long *dNreal = nullptr, *dSigOff = nullptr;
+    double *dSigma = nullptr;
 };
 
 extern "C" {
@@ -1055,7 +1060,7 @@ avs_status avs_batch_destroy(avs_batch *B)
 {
     if (!B) return AVS_OK;
     if (B->stream) hipStreamSynchronize(B->stream);
-    void *ptrs[] = {B->dChase, B->dDE, B->dChaseErr, B->dArena, B->dStage, B->dBands, B->dScratch, B->dOff, B->dSoff, B->dTile0,
+    void *ptrs[] = {B->dWmat, B->dWfirst, B->dNreal, B->dSigOff, B->dSigma, B->dChase, B->dDE, B->dChaseErr, B->dArena, B->dStage, B->dBands, B->dScratch, B->dOff, B->dSoff, B->dTile0,
                     B->dCol0, B->dBoff, B->dNts, B->dOrder, B->dNext};
     for (void *p : ptrs) hipFree(p);
     if (B->ev0) hipEventDestroy(B->ev0);
@@ -1154,8 +1159,9 @@ avs_status avs_batch_get_band(avs_batch *B, int32_t idx, double *band, int64_t l
 }
 
 // Stage 2 on the device: every matrix's band -> bidiagonal in one launch (one
-// wavefront per matrix), then only (d, e) comes back.
-static avs_status batch_device_chase(avs_batch *B)
+// wavefront per matrix), then only (d, e) comes back (fetch_de), or the
+// bidiagonals stay on the device for batch_device_sigma.
+static avs_status batch_device_chase(avs_batch *B, bool fetch_de = true)
 {
     const size_t cols = (size_t)B->total_cols;
     if (!B->dChase) {
@@ -1176,7 +1182,8 @@ static avs_status batch_device_chase(avs_batch *B)
     });
     if (st) return st;
     std::vector<int> errs(B->count);
-    HIPCHK(hipMemcpyAsync(B->hDE, B->dDE, sizeof(double) * 2 * cols, hipMemcpyDeviceToHost, B->stream));
+    if (fetch_de)
+        HIPCHK(hipMemcpyAsync(B->hDE, B->dDE, sizeof(double) * 2 * cols, hipMemcpyDeviceToHost, B->stream));
     HIPCHK(hipMemcpyAsync(errs.data(), B->dChaseErr, sizeof(int) * (size_t)B->count, hipMemcpyDeviceToHost,
                           B->stream));
     HIPCHK(hipStreamSynchronize(B->stream));
@@ -1193,6 +1200,59 @@ static bool batch_stage2_on_device(const avs_batch *B)
     return B->b <= 32 && !(pol && *pol);
 }
 
+// the bidiagonals' singular values on the device too (bisection) unless
+// AVSVD_BDSDC=host asks for LAPACK dbdsdc on the host; the kernel stages
+// 2N - 1 <= 4096 values in LDS
+static bool batch_sigma_on_device(const avs_batch *B)
+{
+    const char *pol = getenv("AVSVD_BDSDC");
+    if (pol && strcmp(pol, "host") == 0) return false;
+    for (int m = 0; m < B->count; ++m)
+        if (2L * B->nt[m] * B->b - 1 > 4096) return false;
+    return true;
+}
+
+// Stage 2 and the singular values on the device; only sigma comes back.
+static avs_status batch_device_sigma(avs_batch *B, const std::vector<long> &soff, double *sigma)
+{
+    avs_status st = batch_device_chase(B, false);
+    if (st) return st;
+    long total = 0;
+    for (int m = 0; m < B->count; ++m) total += B->n[m];
+    if (!B->dWmat) {
+        // workgroups of the largest matrices first; none for a range that
+        // holds only padding values
+        std::vector<int> order(B->count), wm, wf;
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return B->nt[x] > B->nt[y]; });
+        for (int m : order) {
+            const long N = (long)B->nt[m] * B->b;
+            for (long j0 = 0; j0 < N; j0 += BATCH_BISECT_THREADS) {
+                if (j0 + BATCH_BISECT_THREADS <= N - B->n[m]) continue;
+                wm.push_back(m);
+                wf.push_back((int)j0);
+            }
+        }
+        std::vector<long> nreal(B->n.begin(), B->n.end());
+        B->nbis = (int)wm.size();
+        HIPCHK(hipMalloc(&B->dWmat, sizeof(int) * wm.size()));
+        HIPCHK(hipMalloc(&B->dWfirst, sizeof(int) * wf.size()));
+        HIPCHK(hipMalloc(&B->dNreal, sizeof(long) * B->count));
+        HIPCHK(hipMalloc(&B->dSigOff, sizeof(long) * B->count));
+        HIPCHK(hipMalloc(&B->dSigma, sizeof(double) * (size_t)std::max(total, 1L)));
+        HIPCHK(hipMemcpy(B->dWmat, wm.data(), sizeof(int) * wm.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(B->dWfirst, wf.data(), sizeof(int) * wf.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(B->dNreal, nreal.data(), sizeof(long) * B->count, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(B->dSigOff, soff.data(), sizeof(long) * B->count, hipMemcpyHostToDevice));
+    }
+    launch_batch_bisect(B->dDE, B->dCol0, B->dNts, B->b, B->dWmat, B->dWfirst, B->nbis, B->dNreal, B->dSigOff,
+                        B->dSigma, B->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sigma, B->dSigma, sizeof(double) * total, hipMemcpyDeviceToHost, B->stream));
+    HIPCHK(hipStreamSynchronize(B->stream));
+    return AVS_OK;
+}
+
 avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
 {
     if (!B) return fail(-1, "null batch");
@@ -1202,10 +1262,11 @@ avs_status avs_batch_svdvals(avs_batch *B, double *sigma)
         if ((st = avs_batch_blockbidiag(B))) return st;
     }
     B->bands_valid = false;
-    const bool dev2 = batch_stage2_on_device(B);
-    if ((st = dev2 ? batch_device_chase(B) : batch_fetch_bands(B))) return st;
     std::vector<long> soff(B->count, 0);
     for (int i = 1; i < B->count; ++i) soff[i] = soff[i - 1] + B->n[i - 1];
+    const bool dev2 = batch_stage2_on_device(B);
+    if (dev2 && batch_sigma_on_device(B)) return batch_device_sigma(B, soff, sigma);
+    if ((st = dev2 ? batch_device_chase(B) : batch_fetch_bands(B))) return st;
     std::vector<int> order(B->count);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return B->nt[x] > B->nt[y]; });

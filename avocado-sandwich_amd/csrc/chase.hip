@@ -225,7 +225,83 @@ __global__ void __launch_bounds__(CHASE_THREADS) batch_chase_kernel(const TS *__
     }
 }
 
+// Singular values of the bidiagonals by bisection on their Golub-Kahan
+// tridiagonal T (order 2N, zero diagonal, off-diagonals d_0, e_0, d_1, ...,
+// d_{N-1}), whose eigenvalues are +-sigma.  The Sturm count of T - xI (the
+// pivots q_0 = -x, q_k = -x - t_{k-1}^2 / q_{k-1}, each kept at least pivmin
+// away from zero as in LAPACK dstebz) is the number of eigenvalues below x,
+// so for x > 0 the number of singular values below x is count - N.  One lane
+// per singular value: 64 halvings of [0, Gershgorin bound of T] put it within
+// 2^-64 of the bound, below the rounding of the count itself.  Workgroup g
+// handles singular values wfirst[g] .. +BIS_THREADS-1 (ascending) of matrix
+// wmat[g]; t^2 is staged in LDS (2N - 1 <= BIS_LDS); the padding's N - n
+// zero singular values (the smallest) are not written.
+constexpr int BIS_THREADS = 256;
+constexpr int BIS_LDS = 4096;
+
+__global__ void __launch_bounds__(BIS_THREADS) batch_bisect_kernel(const double *__restrict__ de,
+                                                                   const long *__restrict__ col0,
+                                                                   const int *__restrict__ nts, int b,
+                                                                   const int *__restrict__ wmat,
+                                                                   const int *__restrict__ wfirst,
+                                                                   const long *__restrict__ nreal,
+                                                                   const long *__restrict__ soff,
+                                                                   double *__restrict__ sigma)
+{
+    __shared__ double t2[BIS_LDS];
+    __shared__ double red[2][BIS_THREADS];
+    const int m = wmat[blockIdx.x], t = threadIdx.x;
+    const long N = (long)nts[m] * b, L = 2 * N - 1;
+    const double *d = de + 2 * col0[m], *e = d + N;
+    // t_k^2 into LDS; row bound |t_{k-1}| + |t_k| and the largest t^2
+    double bnd = 0.0, tmx = 0.0;
+    for (long k = t; k <= L; k += BIS_THREADS) {
+        const double tk = k < L ? fabs((k & 1) ? e[k >> 1] : d[k >> 1]) : 0.0;
+        const double tp = k > 0 ? fabs(((k - 1) & 1) ? e[(k - 1) >> 1] : d[(k - 1) >> 1]) : 0.0;
+        if (k < L) t2[k] = tk * tk;
+        bnd = fmax(bnd, tk + tp);
+        tmx = fmax(tmx, tk * tk);
+    }
+    red[0][t] = bnd;
+    red[1][t] = tmx;
+    __syncthreads();
+    for (int s = BIS_THREADS / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            red[0][t] = fmax(red[0][t], red[0][t + s]);
+            red[1][t] = fmax(red[1][t], red[1][t + s]);
+        }
+        __syncthreads();
+    }
+    const double hi0 = red[0][0] * (1.0 + 1e-14) + 1e-300;
+    const double pivmin = 2.2250738585072014e-308 * fmax(1.0, red[1][0]);
+    const long j = wfirst[blockIdx.x] + t; // ascending index among the N singular values
+    if (j >= N || j < N - nreal[m]) return;
+    double lo = 0.0, hi = hi0;
+    for (int it = 0; it < 64; ++it) {
+        const double x = 0.5 * (lo + hi);
+        double q = -x;
+        if (fabs(q) < pivmin) q = -pivmin;
+        long cnt = q < 0.0;
+        for (long k = 0; k < L; ++k) {
+            q = -x - t2[k] / q;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < 0.0;
+        }
+        if (cnt - N >= j + 1) hi = x;
+        else lo = x;
+    }
+    sigma[soff[m] + (N - 1 - j)] = 0.5 * (lo + hi);
+}
+
 } // namespace
+
+void launch_batch_bisect(const double *de, const long *col0, const int *nts, int b, const int *wmat,
+                         const int *wfirst, int nwg, const long *nreal, const long *soff, double *sigma,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_bisect_kernel, dim3((unsigned)nwg), dim3(BIS_THREADS), 0, s, de, col0, nts, b, wmat,
+                       wfirst, nreal, soff, sigma);
+}
 
 template <typename TS>
 void launch_batch_chase(const void *arena, const long *off, const int *nts, const int *order, const long *col0,

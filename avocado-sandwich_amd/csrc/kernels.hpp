@@ -44,6 +44,14 @@ template <typename TS>
 void launch_batch_chase(const void *arena, const long *off, const int *nts, const int *order, const long *col0,
                         int count, int b, double *store, double *de, int *err, hipStream_t s);
 
+// singular values of the bidiagonals (de as launch_batch_chase writes it) by
+// bisection, one lane per value; workgroup g: matrix wmat[g], ascending values
+// wfirst[g] .. +255; 2N-1 <= 4096; sigma[soff[m] ..] gets the n_m largest
+constexpr int BATCH_BISECT_THREADS = 256;
+void launch_batch_bisect(const double *de, const long *col0, const int *nts, int b, const int *wmat,
+                         const int *wfirst, int nwg, const long *nreal, const long *soff, double *sigma,
+                         hipStream_t s);
+
 template <typename TS, int B>
 void launch_chain_panel(int nchains, int mode, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
                         const int *chain_start, int chain0, hipStream_t s);

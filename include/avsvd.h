@@ -244,9 +244,12 @@ avs_status avs_batch_get_band(avs_batch *batch, int32_t idx, double *band, int64
 /* All singular values, concatenated in matrix order (n_i values each,
  * descending); runs the reduction first if needed.  For b <= 32 the band ->
  * bidiagonal step runs on the device (one wavefront per matrix, Householder
- * bulge chasing in FP64) and only (d, e) comes back for dbdsdc on the host;
- * AVSVD_STAGE2=host (or dgbbrd, halving) runs that step on the host instead.
- * AVS_ERR_STATE if the device step leaves its band store (bounds-checked). */
+ * bulge chasing in FP64), and so do the bidiagonals' singular values
+ * (bisection on the Golub-Kahan tridiagonal, one lane per value) when
+ * 2N - 1 <= 4096; only sigma comes back.  AVSVD_BDSDC=host computes them with
+ * LAPACK dbdsdc on the host; AVSVD_STAGE2=host (or dgbbrd, halving) runs both
+ * steps on the host.  AVS_ERR_STATE if the device chase leaves its band store
+ * (bounds-checked). */
 avs_status avs_batch_svdvals(avs_batch *batch, double *sigma);
 
 /* ---- serving: continuous-batching svdvals request loop ----

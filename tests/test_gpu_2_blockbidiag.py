@@ -232,29 +232,37 @@ def test_graph_replays_match_eager(gpu, n, b, dt):
 
 
 def test_batched_device_stage2_matches_host(gpu, monkeypatch):
-    """The batch's band -> bidiagonal on the device (one wavefront per matrix,
-    the default for b <= 32) gives the sigma of the host bulge chasing of the
-    same bands, to 10*n*eps of FP64, and both match FP64 LAPACK to the
-    working precision's bar."""
+    """The batch's host stage on the device (the default for b <= 32): band ->
+    bidiagonal by bulge chasing (one wavefront per matrix) and the bidiagonal's
+    singular values by bisection (one lane per value).  Each device step gives
+    the sigma of its host counterpart (host bulge chasing, LAPACK dbdsdc) to
+    10*n*eps of FP64, and all match FP64 LAPACK to that bar."""
     rng = np.random.default_rng(17)
-    ns = [1, 2, 31, 32, 33, 64, 97, 200, 256, 300]
+    ns = [1, 2, 31, 32, 33, 64, 97, 200, 256, 300, 1000]
     mats = [(rng.standard_normal((n, n)) / np.sqrt(n)) for n in ns]
+    mats[6][:, 3] = 0.0  # a rank-deficient matrix: an exact zero singular value
     outs = []
-    for pol in ("", "host"):
-        print("case batched stage 2 policy=%r" % (pol or "device"), flush=True)
-        if pol:
-            monkeypatch.setenv("AVSVD_STAGE2", pol)
-        else:
-            monkeypatch.delenv("AVSVD_STAGE2", raising=False)
+    for stage2, bdsdc in (("", ""), ("", "host"), ("host", "")):
+        print("case batched stage2=%s sigma=%s" % (stage2 or "device", "host" if stage2 or bdsdc else "device"),
+              flush=True)
+        for var, val in (("AVSVD_STAGE2", stage2), ("AVSVD_BDSDC", bdsdc)):
+            if val:
+                monkeypatch.setenv(var, val)
+            else:
+                monkeypatch.delenv(var, raising=False)
         bt = avsvd.Batch(ns, dtype=np.float64, b=32)
         for i, m in enumerate(mats):
             bt.set(i, m)
         outs.append(bt.svdvals())
         del bt
-    for n, m, dv, hs in zip(ns, mats, outs[0], outs[1]):
+    for k, (n, m) in enumerate(zip(ns, mats)):
         bar = 10 * max(n, 2) * np.finfo(np.float64).eps
-        assert O.sigma_err(dv, hs) <= bar, n
-        assert O.sigma_err(dv, sl.svdvals(m)) <= bar, n
+        ref = sl.svdvals(m)
+        for o in outs:
+            assert len(o[k]) == n
+            assert O.sigma_err(o[k], ref) <= bar, n
+        assert O.sigma_err(outs[0][k], outs[1][k]) <= bar, n
+        assert O.sigma_err(outs[1][k], outs[2][k]) <= bar, n
 
 
 def test_serving_results_same_alone_and_in_mix(gpu):

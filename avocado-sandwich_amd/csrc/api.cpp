@@ -11,6 +11,7 @@
 #include <mutex>
 #include <tuple>
 #include <atomic>
+#include <cmath>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -418,6 +419,65 @@ template <typename TS, int B> avs_status run_streams_graph(avs_handle *h)
     return AVS_OK;
 }
 
+} // namespace
+
+namespace avs {
+
+// Singular values of a bidiagonal by bisection on the device (one lane per
+// value), with this thread's stream and buffers (created on first use, grown
+// as needed and kept for the thread's lifetime); LAPACK dbdsdc on the host
+// for n < 128 (the device round trip costs more) or with AVSVD_BDSDC=host.
+avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma)
+{
+    if (n <= 0 || nkeep <= 0 || nkeep > n) return fail(-1, "bad sizes n=%lld nkeep=%lld", (long long)n,
+                                                       (long long)nkeep);
+    const char *pol = getenv("AVSVD_BDSDC");
+    if (n < 128 || (pol && strcmp(pol, "host") == 0)) {
+        std::vector<double> sg(n);
+        avs_status st = avs_bdsdc_values(n, d, e, sg.data());
+        if (st) return st;
+        memcpy(sigma, sg.data(), sizeof(double) * nkeep);
+        return AVS_OK;
+    }
+    thread_local hipStream_t ts = nullptr;
+    thread_local double *dT2 = nullptr, *dSig = nullptr;
+    thread_local int64_t cap = 0;
+    if (!ts) HIPCHK(hipStreamCreateWithFlags(&ts, hipStreamNonBlocking));
+    if (cap < n) {
+        if (dT2) HIPCHK(hipFree(dT2));
+        if (dSig) HIPCHK(hipFree(dSig));
+        dT2 = dSig = nullptr;
+        cap = 0;
+        HIPCHK(hipMalloc(&dT2, sizeof(double) * 2 * (size_t)n));
+        HIPCHK(hipMalloc(&dSig, sizeof(double) * (size_t)n));
+        cap = n;
+    }
+    // squared off-diagonals of the Golub-Kahan tridiagonal, its Gershgorin
+    // bound and LAPACK's pivmin
+    std::vector<double> t2(2 * n - 1);
+    double bnd = 0.0, tmx = 0.0, prev = 0.0;
+    for (int64_t k = 0; k < 2 * n - 1; ++k) {
+        const double t = std::fabs((k & 1) ? e[k >> 1] : d[k >> 1]);
+        t2[k] = t * t;
+        bnd = std::max(bnd, prev + t);
+        tmx = std::max(tmx, t2[k]);
+        prev = t;
+    }
+    bnd = std::max(bnd, prev);
+    const double hi0 = bnd * (1.0 + 1e-14) + 1e-300;
+    const double pivmin = 2.2250738585072014e-308 * std::max(1.0, tmx);
+    HIPCHK(hipMemcpyAsync(dT2, t2.data(), sizeof(double) * t2.size(), hipMemcpyHostToDevice, ts));
+    launch_bisect(dT2, n, nkeep, hi0, pivmin, dSig, ts);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sigma, dSig, sizeof(double) * nkeep, hipMemcpyDeviceToHost, ts));
+    HIPCHK(hipStreamSynchronize(ts));
+    return AVS_OK;
+}
+
+} // namespace avs
+
+namespace {
+
 template <typename TS, int B> avs_status run_persistent(avs_handle *h)
 {
     const Program &P = h->prog->P;
@@ -758,7 +818,7 @@ avs_status avs_svdvals(avs_handle *h, double *sigma)
     if (st) return st;
     st = bidiagonalize_band_auto(n, b, band.data(), b + 1, d.data(), e.data());
     if (st) return st;
-    return avs_bdsdc_values(n, d.data(), e.data(), sigma);
+    return bidiag_sigma(n, n, d.data(), e.data(), sigma);
 }
 
 // ------------------------------------------------------------ tile operators

@@ -293,7 +293,40 @@ __global__ void __launch_bounds__(BIS_THREADS) batch_bisect_kernel(const double 
     sigma[soff[m] + (N - 1 - j)] = 0.5 * (lo + hi);
 }
 
+// One bidiagonal's singular values by the same bisection, t^2 read from HBM
+// (every lane reads the same t2[k] at the same step); lane j of the grid finds
+// the j-th smallest and the nkeep largest are written, descending.
+__global__ void __launch_bounds__(BIS_THREADS) bisect_kernel(const double *__restrict__ t2, long N, long nkeep,
+                                                             double hi0, double pivmin, double *__restrict__ sigma)
+{
+    const long j = (long)blockIdx.x * BIS_THREADS + threadIdx.x;
+    if (j >= N || j < N - nkeep) return;
+    const long L = 2 * N - 1;
+    double lo = 0.0, hi = hi0;
+    for (int it = 0; it < 64; ++it) {
+        const double x = 0.5 * (lo + hi);
+        double q = -x;
+        if (fabs(q) < pivmin) q = -pivmin;
+        long cnt = q < 0.0;
+        for (long k = 0; k < L; ++k) {
+            q = -x - t2[k] / q;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < 0.0;
+        }
+        if (cnt - N >= j + 1) hi = x;
+        else lo = x;
+    }
+    sigma[N - 1 - j] = 0.5 * (lo + hi);
+}
+
 } // namespace
+
+void launch_bisect(const double *t2, long N, long nkeep, double hi0, double pivmin, double *sigma, hipStream_t s)
+{
+    const long grid = (N + BIS_THREADS - 1) / BIS_THREADS;
+    hipLaunchKernelGGL(bisect_kernel, dim3((unsigned)grid), dim3(BIS_THREADS), 0, s, t2, N, nkeep, hi0, pivmin,
+                       sigma);
+}
 
 void launch_batch_bisect(const double *de, const long *col0, const int *nts, int b, const int *wmat,
                          const int *wfirst, int nwg, const long *nreal, const long *soff, double *sigma,

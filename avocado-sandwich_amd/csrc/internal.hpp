@@ -14,6 +14,10 @@ avs_status bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *band, 
 avs_status bidiagonalize_band_one(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,
                                   int nthreads);
 avs_status bidiagonalize_band_auto(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e);
+// singular values of the upper bidiagonal (d, e) of order n, the nkeep
+// largest, descending: bisection on the device (bidiag_sigma in api.cpp), or
+// LAPACK dbdsdc on the host for n < 128 or with AVSVD_BDSDC=host
+avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma);
 // band -> bidiagonal by Householder bulge chasing, sweeps pipelined over
 // nthreads host threads (bandbrd.cpp); the result does not depend on nthreads
 avs_status bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,

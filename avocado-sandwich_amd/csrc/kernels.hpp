@@ -52,6 +52,10 @@ void launch_batch_bisect(const double *de, const long *col0, const int *nts, int
                          const int *wfirst, int nwg, const long *nreal, const long *soff, double *sigma,
                          hipStream_t s);
 
+// one bidiagonal: t2 = squared off-diagonals of its Golub-Kahan tridiagonal
+// (2N - 1 values); sigma[0 .. nkeep) gets the nkeep largest, descending
+void launch_bisect(const double *t2, long N, long nkeep, double hi0, double pivmin, double *sigma, hipStream_t s);
+
 template <typename TS, int B>
 void launch_chain_panel(int nchains, int mode, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
                         const int *chain_start, int chain0, hipStream_t s);

@@ -172,6 +172,10 @@ void host_worker(avs_server *S)
             // requests set the p99; the sweeps pipeline well there)
             avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, d.data(), e.data(),
                                                         N >= 4096 ? 4 : 1);
+            // LAPACK dbdsdc on this host thread: with the device bisection
+            // here (avs::bidiag_sigma, one stream per host worker) a 48-request
+            // run with 16 host workers stopped making progress once, cause
+            // not identified, so the serving loop does not use it
             if (!st) st = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
             if (st) {
                 r->status = st;

@@ -231,6 +231,26 @@ def test_graph_replays_match_eager(gpu, n, b, dt):
         T.synchronize()
 
 
+@pytest.mark.parametrize("n,b,dt", [(1024, 64, np.float64), (640, 32, np.float32), (96, 32, np.float64)])
+def test_svdvals_device_bisection_matches_dbdsdc(gpu, monkeypatch, n, b, dt):
+    """The plain path's last step, the bidiagonal's singular values: bisection
+    on the device (default from n = 128) and LAPACK dbdsdc on the host agree to
+    10*n*eps of FP64, and both match FP64 LAPACK to the working precision's bar."""
+    a = O.gaussian_matrix(n, BASE + 11 * n + b)
+    outs = []
+    for pol in ("", "host"):
+        print("case svdvals n=%d b=%d sigma=%s" % (n, b, pol or "device"), flush=True)
+        if pol:
+            monkeypatch.setenv("AVSVD_BDSDC", pol)
+        else:
+            monkeypatch.delenv("AVSVD_BDSDC", raising=False)
+        outs.append(avsvd.svdvals(a.astype(dt), tilesize=b))
+    assert O.sigma_err(outs[0], outs[1]) <= 10 * n * np.finfo(np.float64).eps
+    ref = sl.svdvals(a.astype(dt).astype(np.float64))
+    for o in outs:
+        assert O.sigma_err(o, ref) <= bar(dt, n)
+
+
 def test_batched_device_stage2_matches_host(gpu, monkeypatch):
     """The batch's host stage on the device (the default for b <= 32): band ->
     bidiagonal by bulge chasing (one wavefront per matrix) and the bidiagonal's

@@ -232,8 +232,10 @@ def suite_serving(args):
     from .autotune import load_table, push_to_library
     from .server import Server, served_tile
     from .tiled import TiledMatrix
-    # the plain path below takes the tile size the serving loop gives each request
+    # the plain path below takes the tile size the serving loop gives each
+    # request and the same last step (LAPACK dbdsdc on the host)
     push_to_library(load_table())
+    os.environ["AVSVD_BDSDC"] = "host"
     rng = np.random.default_rng(SEED + 11)
     reqs = []
     for i in range(args.requests):

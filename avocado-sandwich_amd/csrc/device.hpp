@@ -180,7 +180,9 @@ template <int N> __device__ __forceinline__ double tree_sum(double (&v)[N])
 template <int R> __device__ __forceinline__ double reg_get(const double (&a)[R], int s)
 {
     if constexpr (R <= 16) {
-        return a[s];
+        // the slot in a scalar register: VGPR-indexed moves instead of a
+        // select over all R registers
+        return a[__builtin_amdgcn_readfirstlane(s)];
     } else {
         double v = 0.0;
 #pragma unroll
@@ -191,7 +193,7 @@ template <int R> __device__ __forceinline__ double reg_get(const double (&a)[R],
 template <int R> __device__ __forceinline__ void reg_set(double (&a)[R], int s, double d)
 {
     if constexpr (R <= 16) {
-        a[s] = d;
+        a[__builtin_amdgcn_readfirstlane(s)] = d;
     } else {
 #pragma unroll
         for (int u = 0; u < R; ++u) a[u] = (u == s) ? d : a[u];

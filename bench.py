@@ -43,6 +43,26 @@ def cpu_baseline(n, b):
                       "matrix: %.2f s" % (n, b, dt), "seconds": dt}
 
 
+def panel_rate(n, b, tree, domain, dtype, panel_ms):
+    """Achieved HBM rate of the panel kernels: algorithmic bytes of every panel
+    operation of one reduction (GEQRT reads and writes its tile, 2 b^2 elements;
+    TSQRT and the merges their two tiles' R triangle and bottom tile,
+    2 (b(b+1)/2 + b^2) elements) over the panel launches' summed duration per
+    reduction.  The panels are latency-bound; the rate is low by nature."""
+    import avsvd
+    elt = 8 if dtype == "f64" else 4
+    ops = {0: 0, 1: 0, 2: 0}
+    for kind, _, chains in avsvd.phase_dump(n // b, b, 0, 0, tree, domain or 2):
+        if kind == 0:
+            for ch in chains:
+                for op in ch:
+                    ops[op["mode"]] += 1
+    nbytes = (ops[0] * 2 * b * b + (ops[1] + ops[2]) * 2 * (b * (b + 1) // 2 + b * b)) * elt
+    return {"bound": "latency", "algorithmic_bytes_per_step": nbytes, "panel_ms_per_step": round(panel_ms, 3),
+            "achieved_gbps": round(nbytes / panel_ms / 1e6, 2) if panel_ms > 0 else None, "peak_gbps": 8000.0,
+            "ops": {"ge": ops[0], "ts": ops[1], "tt": ops[2]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +239,7 @@ def main():
                      if upd["launches"] else None,
                      "panel_ms_per_step": round(upd["panel_ms"] / args.steps, 3),
                      "step_kernel_ms": [round(x, 3) for x in kernel_ms]},
+        "panels": panel_rate(n, b, o.tree, args.domain, args.dtype, upd["panel_ms"] / args.steps),
         "band_fro_rel_err": fro_err,
         "sigma_check": sigma_check,
         "cpu_baseline": cpu,

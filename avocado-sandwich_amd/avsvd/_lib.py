@@ -23,7 +23,7 @@ EXPORTS = (
     "avs_default_opts", "avs_create", "avs_destroy", "avs_set_matrix_host", "avs_set_matrix_device",
     "avs_get_matrix_host", "avs_get_matrix_device", "avs_blockbidiag", "avs_tiled_qr", "avs_get_band",
     "avs_svdvals", "avs_synchronize", "avs_last_time_ms", "avs_last_error", "avs_bidiagonalize_band",
-    "avs_bidiagonalize_band_lapack", "avs_bidiagonalize_band_chase",
+    "avs_bidiagonalize_band_lapack", "avs_bidiagonalize_band_chase", "avs_bidiagonalize_band_device",
     "avs_bdsdc_values", "avs_geqrt", "avs_tsqrt", "avs_unmqr", "avs_tsmqr", "avs_gen_gaussian_host",
     "avs_debug_mfma_layout", "avs_schedule_stats", "avs_schedule_dump", "avs_phase_dump", "avs_phase_timing", "avs_phase_trace",
     "avs_batch_create", "avs_batch_destroy", "avs_batch_set_matrix_host", "avs_batch_blockbidiag",
@@ -90,6 +90,7 @@ def lib():
             "avs_bidiagonalize_band": [i64, i32, dp, i64, dp, dp],
             "avs_bidiagonalize_band_lapack": [i64, i32, dp, i64, dp, dp],
             "avs_bidiagonalize_band_chase": [i64, i32, dp, i64, dp, dp],
+            "avs_bidiagonalize_band_device": [i64, i32, dp, i64, dp, dp],
             "avs_bdsdc_values": [i64, dp, dp, dp],
             "avs_geqrt": [i32, i32, vp, i64, vp, i32, vp],
             "avs_tsqrt": [i32, i32, vp, i64, vp, i64, vp, i32, vp],

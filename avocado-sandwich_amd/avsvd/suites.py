@@ -1,6 +1,7 @@
 """Benchmark suites for the workloads the band reduction is quoted on.
 
     python -m avsvd.suites plain     [--ns 4096 8192 16384] [--bs 32 64 128] [--dtypes f64 f32] [--kinds gauss geom]
+    python -m avsvd.suites stage2    [--ns 4096 8192 16384] [--bs 32 64 128]
     python -m avsvd.suites precision [--n 16384] [--conds 1e2 1e4 1e6]
     python -m avsvd.suites batched   [--count 4096] [--plain-sample 128]
     python -m avsvd.suites serving   [--requests 48] [--budget-gb 64]
@@ -118,6 +119,40 @@ def suite_plain(args):
                     emit(d)
                 del t
                 torch.cuda.empty_cache()
+
+
+def suite_stage2(args):
+    """Band -> bidiagonal -> sigma after one FP64 band reduction, with the
+    chase on the device (one launch per time of the pipelined schedule) and on
+    the host (sweeps pipelined over the host threads): seconds of avs_svdvals
+    after the reduction (band store or band D2H, chase, device bisection)."""
+    import torch
+    for n in args.ns:
+        t, ref, rlabel = device_matrix(n, "f64", "gauss", SEED, 1e6)
+        torch.cuda.synchronize()
+        for b in args.bs:
+            R = Reducer(n, b, "f64")
+            ms = R.reduce(t.data_ptr(), reps=1)
+            d = {"suite": "stage2", "n": n, "b": b, "reduction_s": round(ms / 1e3, 4)}
+            sig = {}
+            old = os.environ.get("AVSVD_STAGE2")
+            try:
+                for pol in ("device", "host"):
+                    os.environ["AVSVD_STAGE2"] = pol
+                    sig[pol], sec = R.svdvals()
+                    d[pol + "_s"] = round(sec, 4)
+            finally:
+                if old is None:
+                    os.environ.pop("AVSVD_STAGE2", None)
+                else:
+                    os.environ["AVSVD_STAGE2"] = old
+            R.close()
+            d["device_vs_host_sigma_err"] = sigma_err(sig["device"], sig["host"])
+            if ref is not None:
+                d["device_sigma_err_over_sigma_max"] = sigma_err(sig["device"], ref)
+                d["reference"] = rlabel
+            print(json.dumps(d), flush=True)
+        del t
 
 
 def suite_precision(args):
@@ -329,6 +364,9 @@ def main(argv=None):
     p.add_argument("--bs", type=int, nargs="+", default=[32, 64, 128])
     p.add_argument("--dtypes", nargs="+", default=["f64", "f32"])
     p.add_argument("--kinds", nargs="+", default=["gauss", "geom"])
+    p = sub.add_parser("stage2")
+    p.add_argument("--ns", type=int, nargs="+", default=[4096, 8192, 16384])
+    p.add_argument("--bs", type=int, nargs="+", default=[32, 64, 128])
     p = sub.add_parser("precision")
     p.add_argument("--n", type=int, default=16384)
     p.add_argument("--b", type=int, default=128)
@@ -343,7 +381,7 @@ def main(argv=None):
     p.add_argument("--budget-gb", type=float, default=64)
     p.add_argument("--inflight", type=int, default=4)
     args = ap.parse_args(argv)
-    {"plain": suite_plain, "precision": suite_precision, "batched": suite_batched,
+    {"plain": suite_plain, "stage2": suite_stage2, "precision": suite_precision, "batched": suite_batched,
      "serving": suite_serving}[args.suite](args)
 
 

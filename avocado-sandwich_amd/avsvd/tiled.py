@@ -183,6 +183,17 @@ def bidiagonalize_band_chase(band, n, bw):
     return d, e[: n - 1]
 
 
+def bidiagonalize_band_device(band, n, bw):
+    """Band -> (d, e) by the same bulge chasing on the GPU (bw <= 128): the
+    steps of many sweeps at once, one launch per time of the pipelined
+    schedule (what svdvals runs from n = 16384 up)."""
+    band = np.asfortranarray(band, dtype=np.float64)
+    d = np.empty(n)
+    e = np.empty(max(n - 1, 1))
+    check(lib().avs_bidiagonalize_band_device(n, bw, dptr(band), band.shape[0], dptr(d), dptr(e)))
+    return d, e[: n - 1]
+
+
 def _dense_to_band(A, bw):
     n = A.shape[0]
     band = np.zeros((bw + 1, n), order="F")

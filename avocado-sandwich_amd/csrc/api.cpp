@@ -118,6 +118,10 @@ struct avs_handle {
     void *dStage = nullptr; // column-major staging (n*n elements)
     double *dTau = nullptr, *dTjj = nullptr, *dBand = nullptr;
     double *hBand = nullptr; // pinned host staging for the band (b+1) x n
+    // band store ((3b+1) x n), its diagonal and superdiagonal, and the access
+    // flag of the chase on the device
+    double *dStore = nullptr, *dDE = nullptr;
+    int *dChaseErr = nullptr;
     int *dCnt = nullptr, *dTicket = nullptr, *dStatus = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool have_matrix = false;
@@ -474,9 +478,82 @@ avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double 
     return AVS_OK;
 }
 
+// Band -> bidiagonal of one matrix on the device, from a band store (ld 3b+1,
+// rows c-2b .. c+b of column c, the band in place and zeros elsewhere, n +
+// CHASE_SPARE_COLS columns allocated, err cleared) on stream s: one chase_step launch per time of the pipelined
+// schedule (step s of sweep i at time 3i + s; the steps of one time touch
+// disjoint parts of the store), then d and e to the host.
+avs_status chase_store_on_device(double *dStore, int64_t n, int b, int *dErr, double *dDE, double *d, double *e,
+                                 hipStream_t s)
+{
+    if (b > 128) return fail(AVS_ERR_UNSUPPORTED, "device chase: bandwidth %d > 128", b);
+    const long last = n - 1, nsweeps = n - 1;
+    if (b >= 2 && nsweeps >= 1) {
+        // steps of sweep i: T1 + T2, then one T3 + T4 per r = i+1, i+1+b, ...
+        // with r + b < last (Chase::steps of the host routine in closed form)
+        std::vector<long> steps(nsweeps);
+        for (long i = 0; i < nsweeps; ++i) {
+            const long x = last - i - 1 - b;
+            steps[i] = 1 + (x > 0 ? (x + b - 1) / b : 0);
+        }
+        // sweeps ilo .. ihi are active at time t: started (3i <= t) and not
+        // finished (t - 3i < steps[i]); finished sweeps form a prefix
+        long ilo = 0;
+        for (long t = 0; ilo < nsweeps; ++t) {
+            const long ihi = std::min(t / 3, nsweeps - 1);
+            while (ilo <= ihi && t - 3 * ilo >= steps[ilo]) ++ilo;
+            if (ilo <= ihi) launch_chase_step(dStore, n, b, t, ilo, ihi, dErr, s);
+        }
+    }
+    launch_chase_diag(dStore, n, b, dDE, s);
+    HIPCHK(hipGetLastError());
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(d, dDE, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+    if (n > 1) HIPCHK(hipMemcpyAsync(e, dDE + n, sizeof(double) * (size_t)(n - 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&herr, dErr, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return fail(AVS_ERR_STATE, "device chase: a band store access outside the band");
+    return AVS_OK;
+}
+
 } // namespace avs
 
 namespace {
+
+// the band of the handle's reduced matrix into its band store, then the chase
+avs_status chase_on_device(avs_handle *h, double *d, double *e)
+{
+    const int64_t n = h->n;
+    const int b = h->b;
+    const size_t ld = 3 * (size_t)b + 1, cols = (size_t)n + CHASE_SPARE_COLS;
+    if (!h->dStore) HIPCHK(hipMalloc(&h->dStore, sizeof(double) * ld * cols));
+    if (!h->dDE) HIPCHK(hipMalloc(&h->dDE, sizeof(double) * 2 * (size_t)n));
+    if (!h->dChaseErr) HIPCHK(hipMalloc(&h->dChaseErr, sizeof(int)));
+    HIPCHK(hipMemsetAsync(h->dStore, 0, sizeof(double) * ld * cols, h->stream));
+    HIPCHK(hipMemsetAsync(h->dChaseErr, 0, sizeof(int), h->stream));
+    // band[(b + i - j) + j*ldband] = A(i, j) lands on store element (i, j) at
+    // j*ld + 2b + i - j with band = store + b and ldband = ld
+    if (h->dtype == AVS_F64) launch_band<double>(h->dTiles, h->dStore + b, (long)ld, h->nt, b, h->stream);
+    else launch_band<float>(h->dTiles, h->dStore + b, (long)ld, h->nt, b, h->stream);
+    HIPCHK(hipGetLastError());
+    return chase_store_on_device(h->dStore, n, b, h->dChaseErr, h->dDE, d, e, h->stream);
+}
+
+// Where avs_svdvals runs the band -> bidiagonal stage: AVSVD_STAGE2=device on
+// the device; any other value (host, halving, dgbbrd) on the host; unset, on
+// the device from n = AVSVD_DEVICE_CHASE_MIN_N (default 16384) up.  Each time
+// of the device schedule is one launch of about 15 us (b = 32) to 50 us
+// (b = 128) on MI355X, and there are about 3n of them, so below n = 16384 the
+// host chase on 16 threads is faster (n = 16384: 0.97 vs 0.98 s at b = 32,
+// 1.29 vs 1.53 s at b = 64, 2.57 vs 2.59 s at b = 128).
+bool svdvals_chase_on_device(int64_t n, int b)
+{
+    if (b < 2 || b > 128) return false;
+    if (const char *p = getenv("AVSVD_STAGE2")) return strcmp(p, "device") == 0;
+    int64_t nmin = 16384;
+    if (const char *p = getenv("AVSVD_DEVICE_CHASE_MIN_N")) nmin = atoll(p);
+    return n >= nmin;
+}
 
 template <typename TS, int B> avs_status run_persistent(avs_handle *h)
 {
@@ -611,6 +688,9 @@ avs_status avs_destroy(avs_handle *h)
     hipFree(h->dTjj);
     hipFree(h->dBand);
     if (h->hBand) hipHostFree(h->hBand);
+    hipFree(h->dStore);
+    hipFree(h->dDE);
+    hipFree(h->dChaseErr);
     hipFree(h->dCnt);
     hipFree(h->dTicket);
     hipFree(h->dStatus);
@@ -801,6 +881,42 @@ avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband)
     return check_status(h);
 }
 
+avs_status avs_bidiagonalize_band_device(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d,
+                                         double *e)
+{
+    if (n <= 0) return fail(-1, "n must be positive");
+    if (bw < 1 || bw > 128) return fail(-2, "bandwidth must be in 1 .. 128");
+    if (!band) return fail(-3, "null band");
+    if (ldband < bw + 1) return fail(-4, "ldband < bw+1");
+    if (!d) return fail(-5, "null d");
+    if (n > 1 && !e) return fail(-6, "null e");
+    const int64_t b = bw, ld = 3 * b + 1;
+    // the band store on the host: A(r, c) at c*ld + 2b + r - c for c-b <= r <= c
+    std::vector<double> S((size_t)(ld * (n + CHASE_SPARE_COLS)), 0.0);
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t r = std::max<int64_t>(0, c - b); r <= c; ++r) S[c * ld + 2 * b + r - c] = band[(b + r - c) + c * ldband];
+    hipStream_t s = nullptr;
+    double *dS = nullptr, *dDE = nullptr;
+    int *dErr = nullptr;
+    avs_status st = AVS_OK;
+    auto run = [&]() -> avs_status {
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIPCHK(hipMalloc(&dS, sizeof(double) * S.size()));
+        HIPCHK(hipMalloc(&dDE, sizeof(double) * 2 * (size_t)n));
+        HIPCHK(hipMalloc(&dErr, sizeof(int)));
+        HIPCHK(hipMemcpyAsync(dS, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(dErr, 0, sizeof(int), s));
+        return chase_store_on_device(dS, n, bw, dErr, dDE, d, e, s);
+    };
+    st = run();
+    if (s) hipStreamSynchronize(s);
+    hipFree(dS);
+    hipFree(dDE);
+    hipFree(dErr);
+    if (s) hipStreamDestroy(s);
+    return st;
+}
+
 avs_status avs_svdvals(avs_handle *h, double *sigma)
 {
     if (!h) return fail(-1, "null handle");
@@ -813,10 +929,15 @@ avs_status avs_svdvals(avs_handle *h, double *sigma)
     }
     const int64_t n = h->n;
     const int b = h->b;
-    std::vector<double> band((size_t)(b + 1) * n), d(n), e(n > 1 ? n - 1 : 1);
-    st = avs_get_band(h, band.data(), b + 1);
-    if (st) return st;
-    st = bidiagonalize_band_auto(n, b, band.data(), b + 1, d.data(), e.data());
+    std::vector<double> d(n), e(n > 1 ? n - 1 : 1);
+    if (svdvals_chase_on_device(n, b)) {
+        st = chase_on_device(h, d.data(), e.data());
+    } else {
+        std::vector<double> band((size_t)(b + 1) * n);
+        st = avs_get_band(h, band.data(), b + 1);
+        if (st) return st;
+        st = bidiagonalize_band_auto(n, b, band.data(), b + 1, d.data(), e.data());
+    }
     if (st) return st;
     return bidiag_sigma(n, n, d.data(), e.data(), sigma);
 }

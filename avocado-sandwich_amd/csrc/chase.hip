@@ -319,6 +319,210 @@ __global__ void __launch_bounds__(BIS_THREADS) bisect_kernel(const double *__res
     sigma[N - 1 - j] = 0.5 * (lo + hi);
 }
 
+// ---------------------------------------------------------------------------
+// One large matrix: the same chase with the steps of many sweeps in flight.
+// Step s of sweep i (s = 0: T1 + T2 on rows i .. i+b; s >= 1: T3 + T4 at
+// r = i+1+(s-1)b) touches rows and columns that step s+3 of sweep i-1 and
+// every later step of earlier sweeps leave alone, so with the time of (i, s)
+// set to 3i + s, the steps of one time are disjoint and each element sees its
+// updates in sweep order, as in the sequential loop.  One launch per time;
+// workgroup k runs step tstep - 3i of sweep i = ihi - k.  A phase (one
+// reflector and its update) costs one round trip to the store and two
+// barriers: every thread first requests its part of the update (at most 2b-1
+// rows for right reflectors, two threads per row; at most 2b-1 columns for
+// left ones, a wavefront per column with lanes along it; all loads in flight
+// together), while wave 0 loads the annihilated segment (two elements per
+// lane, m <= 128) and forms the reflector with lane exchanges; a barrier
+// publishes v and tau through LDS, and the threads update.  b <= 128.
+// ---------------------------------------------------------------------------
+constexpr int WCH_THREADS = 512;
+constexpr int WCH_HALF = 64; // elements of a line per thread
+
+// The reflector of a phase, on wave 0 (two elements per lane, m <= 128):
+// RIGHT annihilates row p over columns c0 .. c0+m-1, LEFT column c0 over rows
+// c0 .. c0+m-1.  Householder vector as wave_house (LAPACK dlarfg convention,
+// divisions; tau = 0 and nothing written when the rest of the segment is
+// zero): v to vsh, tau to sc2[0], and the segment becomes (beta, 0, ..., 0).
+// No update line holds the segment, so it is written before the barrier.
+template <bool RIGHT>
+__device__ __forceinline__ void wave0_reflector(const BandStore &A, long p, long c0, int m, double *vsh, double *sc2)
+{
+    const int t = threadIdx.x;
+    if (t >= 64) return;
+    const int k0 = t, k1 = t + 64;
+    auto seg = [&](int k) { return RIGHT ? A.get(p, c0 + k) : A.get(c0 + k, c0); };
+    auto put = [&](int k, double v) {
+        if (RIGHT) A.set(p, c0 + k, v);
+        else A.set(c0 + k, c0, v);
+    };
+    const double x0 = k0 < m ? seg(k0) : 0.0, x1 = k1 < m ? seg(k1) : 0.0;
+    const double alpha = __shfl(x0, 0, 64);
+    double sc = wave_max(fmax((k0 >= 1 && k0 < m) ? fabs(x0) : 0.0, k1 < m ? fabs(x1) : 0.0));
+    double tau = 0.0;
+    if (sc != 0.0) {
+        sc = fmax(sc, fabs(alpha));
+        const double u0 = x0 / sc, u1 = x1 / sc;
+        const double nrm = sc * sqrt(wave_sum(u0 * u0 + u1 * u1));
+        const double beta = alpha >= 0.0 ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        const double den = alpha - beta;
+        if (k0 < m) {
+            vsh[k0] = k0 == 0 ? 1.0 : x0 / den;
+            put(k0, k0 == 0 ? beta : 0.0);
+        }
+        if (k1 < m) {
+            vsh[k1] = x1 / den;
+            put(k1, 0.0);
+        }
+    }
+    if (t == 0) sc2[0] = tau;
+}
+
+// Right phase: the reflector of row p over columns c0 .. c0+m-1, applied to
+// rows p+1 .. lend.  Two threads per row (element (row, c0+k) at
+// index(row, c0) + k*(ld-1), consecutive rows in consecutive lane pairs), each
+// holding up to 64 elements of its row's in-store interval [klo, khi]; all
+// loads go out before the reflector is known.  The loads past a thread's last
+// element read other elements of the store allocation (CHASE_SPARE_COLS spare
+// columns at its end keep base + 127*(ld-1) inside it) and are not used.
+__device__ __forceinline__ void wg_right_phase(const BandStore &A, long p, long c0, int m, long lend, double *vsh,
+                                               double *sc2)
+{
+    const int t = threadIdx.x, line = t >> 1, h = t & 1;
+    const long ln = p + 1 + line;
+    const bool valid = ln <= lend;
+    const long base = A.index(ln, c0), stride = A.ld - 1;
+    const long klo = max(0L, ln - c0 - A.b);
+    const long khi = min(min((long)m - 1, A.last - c0), ln - c0 + A.ku);
+    if (valid && (klo > 0 || khi < m - 1)) *A.err = 1;
+    const int kb = max((int)klo, WCH_HALF * h), ke = min((int)khi, WCH_HALF * h + WCH_HALF - 1);
+    const bool mine = valid && kb <= ke;
+    double a[WCH_HALF];
+    if (mine) {
+        const double *src = A.S + base;
+#pragma unroll
+        for (int u = 0; u < WCH_HALF; ++u) a[u] = src[(long)(kb + u) * stride];
+    }
+    wave0_reflector<true>(A, p, c0, m, vsh, sc2);
+    __syncthreads();
+    const double tau = sc2[0];
+    if (tau != 0.0) {
+        double w = 0.0;
+        if (mine) {
+#pragma unroll
+            for (int u = 0; u < WCH_HALF; ++u) w = kb + u <= ke ? fma(a[u], vsh[min(kb + u, ke)], w) : w;
+        }
+        // the row's dot product, the same in both threads: lower half + upper half
+        const double o = __shfl_xor(w, 1, 64);
+        const double f = tau * (h == 0 ? w + o : o + w);
+        if (mine) {
+            double *dst = A.S + base;
+#pragma unroll
+            for (int u = 0; u < WCH_HALF; ++u)
+                if (kb + u <= ke) dst[(long)(kb + u) * stride] = fma(-f, vsh[kb + u], a[u]);
+        }
+    }
+    __syncthreads();
+}
+
+// Left phase: the reflector of column c0 over rows c0 .. c0+m-1, applied to
+// columns c0+1 .. lend.  Wave w takes columns c0+1+w, c0+1+w+8, ... (at most
+// 32, as 2b-1 <= 255); lane l holds elements k = l and l+64 of each, so the
+// loads and stores of a column are contiguous (element (c0+k, c) at
+// index(c0, c) + k) and its dot product is a wavefront sum.  All loads go out
+// before the reflector is known; those outside a column's in-store interval
+// [klo, khi] read other elements of the store allocation (index(c0, c) + 127
+// stays inside it) and are not used.
+__device__ __forceinline__ void wg_left_phase(const BandStore &A, long c0, int m, long lend, double *vsh,
+                                              double *sc2)
+{
+    constexpr int NW = WCH_THREADS / 64, CPW = 32;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double a[CPW][2];
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) {
+        const long c = c0 + 1 + wv + (long)NW * j;
+        if (c <= lend) {
+            const double *src = A.S + A.index(c0, c);
+            a[j][0] = src[lane];
+            a[j][1] = src[lane + 64];
+        }
+    }
+    wave0_reflector<false>(A, c0, c0, m, vsh, sc2);
+    __syncthreads();
+    const double tau = sc2[0];
+    if (tau != 0.0) {
+        const double v0 = vsh[lane], v1 = vsh[lane + 64];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            const long c = c0 + 1 + wv + (long)NW * j;
+            if (c > lend) continue; // wave-uniform
+            const long klo = max(max(0L, c - A.ku - c0), -c0);
+            const long khi = min(min((long)m - 1, c + A.b - c0), A.last - c0);
+            if (lane == 0 && (klo > 0 || khi < m - 1)) *A.err = 1;
+            const bool in0 = lane >= klo && lane <= khi, in1 = lane + 64 >= klo && lane + 64 <= khi;
+            double pr = in0 ? a[j][0] * v0 : 0.0;
+            pr = in1 ? fma(a[j][1], v1, pr) : pr;
+            const double f = tau * wave_sum(pr);
+            double *dst = A.S + A.index(c0, c);
+            if (in0) dst[lane] = fma(-f, v0, a[j][0]);
+            if (in1) dst[lane + 64] = fma(-f, v1, a[j][1]);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(WCH_THREADS) chase_step_kernel(double *store, long N, int bw, long tstep, long ihi,
+                                                                 int *err)
+{
+    __shared__ double vsh[2 * WCH_HALF];
+    __shared__ double sc2[2];
+    const long b = bw;
+    BandStore A;
+    A.ld = 3 * b + 1;
+    A.ku = 2 * b;
+    A.b = b;
+    A.last = N - 1;
+    A.S = store;
+    A.err = err;
+    const long last = A.last;
+    const long i = ihi - blockIdx.x, s = tstep - 3 * i;
+    if (i < 0 || s < 0) {
+        if (threadIdx.x == 0) *err = 1;
+        return;
+    }
+    if (s == 0) {
+        // T1: row i beyond the superdiagonal, applied to rows i+1 .. i+b;
+        // T2: column i+1 below the diagonal, applied to columns i+2 .. i+2b
+        const long c0 = i + 1, c1 = min(i + b, last);
+        if (c1 > c0) wg_right_phase(A, i, c0, (int)(c1 - c0 + 1), min(i + b, last), vsh, sc2);
+        const long col = i + 1, r1 = min(i + b, last);
+        if (r1 > col) wg_left_phase(A, col, (int)(r1 - col + 1), min(col + 2 * b - 1, last), vsh, sc2);
+        return;
+    }
+    // T3 on row r beyond its band, T4 on column r+b below the diagonal
+    const long r = i + 1 + (s - 1) * b;
+    const long ca = r + b, cb = min(r + 2 * b - 1, last);
+    if (cb <= ca) {
+        if (threadIdx.x == 0) *err = 1; // the host schedules only existing steps
+        return;
+    }
+    wg_right_phase(A, r, ca, (int)(cb - ca + 1), min(r + 2 * b - 1, last), vsh, sc2);
+    const long r1 = min(ca + b - 1, last);
+    if (r1 > ca) wg_left_phase(A, ca, (int)(r1 - ca + 1), min(ca + 2 * b - 1, last), vsh, sc2);
+}
+
+// de[c] = A(c, c), de[N + c] = A(c, c+1) from the store
+__global__ void __launch_bounds__(256) chase_diag_kernel(const double *__restrict__ store, long N, int bw,
+                                                         double *__restrict__ de)
+{
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const long ld = 3L * bw + 1, ku = 2L * bw;
+    de[c] = store[c * ld + ku];
+    if (c + 1 < N) de[N + c] = store[(c + 1) * ld + ku - 1];
+}
+
 } // namespace
 
 void launch_bisect(const double *t2, long N, long nkeep, double hi0, double pivmin, double *sigma, hipStream_t s)
@@ -334,6 +538,17 @@ void launch_batch_bisect(const double *de, const long *col0, const int *nts, int
 {
     hipLaunchKernelGGL(batch_bisect_kernel, dim3((unsigned)nwg), dim3(BIS_THREADS), 0, s, de, col0, nts, b, wmat,
                        wfirst, nreal, soff, sigma);
+}
+
+void launch_chase_step(double *store, long N, int b, long tstep, long ilo, long ihi, int *err, hipStream_t s)
+{
+    hipLaunchKernelGGL(chase_step_kernel, dim3((unsigned)(ihi - ilo + 1)), dim3(WCH_THREADS), 0, s, store, N, b,
+                       tstep, ihi, err);
+}
+
+void launch_chase_diag(const double *store, long N, int b, double *de, hipStream_t s)
+{
+    hipLaunchKernelGGL(chase_diag_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, store, N, b, de);
 }
 
 template <typename TS>

@@ -44,6 +44,16 @@ template <typename TS>
 void launch_batch_chase(const void *arena, const long *off, const int *nts, const int *order, const long *col0,
                         int count, int b, double *store, double *de, int *err, hipStream_t s);
 
+// band -> bidiagonal of one matrix in a band store (ld 3b+1, rows c-2b .. c+b
+// of column c, chase.hip), b <= 128: the steps of sweeps ilo .. ihi at time
+// tstep (step tstep - 3i of sweep i), one workgroup each.  The store's
+// allocation holds N + CHASE_SPARE_COLS columns (the spare ones are read, never
+// used or written).
+constexpr long CHASE_SPARE_COLS = 128;
+void launch_chase_step(double *store, long N, int b, long tstep, long ilo, long ihi, int *err, hipStream_t s);
+// de[0 .. N) = diagonal, de[N .. 2N-1) = superdiagonal of the store
+void launch_chase_diag(const double *store, long N, int b, double *de, hipStream_t s);
+
 // singular values of the bidiagonals (de as launch_batch_chase writes it) by
 // bisection, one lane per value; workgroup g: matrix wmat[g], ascending values
 // wfirst[g] .. +255; 2N-1 <= 4096; sigma[soff[m] ..] gets the n_m largest

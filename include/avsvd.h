@@ -187,6 +187,20 @@ avs_status avs_bidiagonalize_band_lapack(int64_t n, int32_t bw, const double *ba
  * Argument errors as avs_bidiagonalize_band. */
 avs_status avs_bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d,
                                         double *e);
+/* avs_bidiagonalize_band_device: the same chase on the calling thread's
+ * current GPU, bw <= 128.  The band goes to a band store in HBM, and the
+ * steps of many sweeps run at once: step s of sweep i at time 3i + s, one
+ * launch per time, one workgroup per step (the steps of one time touch
+ * disjoint rows or columns, so every element sees its updates in sweep
+ * order, as on the host); only d and e come back.  Same rounding class as the
+ * host chase, not the same bits.  avs_svdvals runs this stage on the device
+ * for n >= AVSVD_DEVICE_CHASE_MIN_N (default 16384, where it matches or beats
+ * the host chase on 16 threads) and 2 <= b <= 128, or always with
+ * AVSVD_STAGE2=device; AVSVD_STAGE2=host (or halving, dgbbrd) keeps it on the
+ * host.  No reference counterpart.  Argument errors as
+ * avs_bidiagonalize_band; -2 also for bw > 128. */
+avs_status avs_bidiagonalize_band_device(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d,
+                                         double *e);
 /* diag_lapack, KAbasedSVD/src/bulgechasing.jl:50-53: dbdsdc('U','N').
  * sigma[n] descending.  LAPACK is loaded at run time (AVSVD_LAPACK or the
  * OpenBLAS bundled with scipy). */

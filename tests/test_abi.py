@@ -144,6 +144,14 @@ def test_host_stage_argument_errors():
     assert L.avs_bidiagonalize_band_chase(4, 0, avsvd._lib.dptr(d), 5, avsvd._lib.dptr(d),
                                           avsvd._lib.dptr(d)) == -2
     assert L.avs_bidiagonalize_band_chase(4, 4, avsvd._lib.dptr(d), 5, avsvd._lib.dptr(d), None) == -6
+    # the device chase checks its arguments before any device call
+    assert L.avs_bidiagonalize_band_device(0, 4, avsvd._lib.dptr(d), 5, avsvd._lib.dptr(d),
+                                           avsvd._lib.dptr(d)) == -1
+    assert L.avs_bidiagonalize_band_device(4, 129, avsvd._lib.dptr(d), 130, avsvd._lib.dptr(d),
+                                           avsvd._lib.dptr(d)) == -2
+    assert L.avs_bidiagonalize_band_device(4, 4, avsvd._lib.dptr(d), 3, avsvd._lib.dptr(d),
+                                           avsvd._lib.dptr(d)) == -4
+    assert L.avs_bidiagonalize_band_device(4, 4, avsvd._lib.dptr(d), 5, avsvd._lib.dptr(d), None) == -6
 
 
 def test_reference_api_mirror_on_host():

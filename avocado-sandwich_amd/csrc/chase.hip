@@ -293,18 +293,27 @@ __global__ void __launch_bounds__(BIS_THREADS) batch_bisect_kernel(const double 
     sigma[soff[m] + (N - 1 - j)] = 0.5 * (lo + hi);
 }
 
-// One bidiagonal's singular values by the same bisection, t^2 read from HBM
-// (every lane reads the same t2[k] at the same step); lane j of the grid finds
-// the j-th smallest and the nkeep largest are written, descending.
+// One bidiagonal's singular values by the same Sturm counts, t^2 read from
+// HBM (every lane reads the same t2[k] at the same step), four lanes per
+// singular value: in each of 32 rounds lanes 0..2 of the group count at the
+// quarter points of [lo, hi], the counts meet through lane exchanges, and all
+// four lanes keep the same quarter (4^-32 = 2^-64 of the Gershgorin bound, as
+// 64 halvings, in half the sequential rounds and on four times the lanes).
+// Group j of the grid finds the j-th smallest; the nkeep largest are written,
+// descending.
+constexpr int BIS_LANES = 4;
 __global__ void __launch_bounds__(BIS_THREADS) bisect_kernel(const double *__restrict__ t2, long N, long nkeep,
                                                              double hi0, double pivmin, double *__restrict__ sigma)
 {
-    const long j = (long)blockIdx.x * BIS_THREADS + threadIdx.x;
+    const long g = (long)blockIdx.x * BIS_THREADS + threadIdx.x;
+    const long j = g / BIS_LANES; // the same for the four lanes of a group
+    const int sub = (int)(g % BIS_LANES);
     if (j >= N || j < N - nkeep) return;
     const long L = 2 * N - 1;
     double lo = 0.0, hi = hi0;
-    for (int it = 0; it < 64; ++it) {
-        const double x = 0.5 * (lo + hi);
+    for (int it = 0; it < 32; ++it) {
+        const double w = hi - lo;
+        const double x = lo + w * (0.25 * (double)(min(sub, 2) + 1));
         double q = -x;
         if (fabs(q) < pivmin) q = -pivmin;
         long cnt = q < 0.0;
@@ -313,10 +322,19 @@ __global__ void __launch_bounds__(BIS_THREADS) bisect_kernel(const double *__res
             if (fabs(q) < pivmin) q = -pivmin;
             cnt += q < 0.0;
         }
-        if (cnt - N >= j + 1) hi = x;
-        else lo = x;
+        // quarter point s lies at or above the j-th smallest value when more
+        // than j singular values are below it
+        const bool above = cnt - N >= j + 1;
+        const bool a0 = __shfl((int)above, 0, BIS_LANES) != 0;
+        const bool a1 = __shfl((int)above, 1, BIS_LANES) != 0;
+        const bool a2 = __shfl((int)above, 2, BIS_LANES) != 0;
+        const double x0 = lo + w * 0.25, x1 = lo + w * 0.5, x2 = lo + w * 0.75;
+        if (a0) hi = x0;
+        else if (a1) { lo = x0; hi = x1; }
+        else if (a2) { lo = x1; hi = x2; }
+        else lo = x2;
     }
-    sigma[N - 1 - j] = 0.5 * (lo + hi);
+    if (sub == 0) sigma[N - 1 - j] = 0.5 * (lo + hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,7 +545,7 @@ __global__ void __launch_bounds__(256) chase_diag_kernel(const double *__restric
 
 void launch_bisect(const double *t2, long N, long nkeep, double hi0, double pivmin, double *sigma, hipStream_t s)
 {
-    const long grid = (N + BIS_THREADS - 1) / BIS_THREADS;
+    const long grid = (N * BIS_LANES + BIS_THREADS - 1) / BIS_THREADS;
     hipLaunchKernelGGL(bisect_kernel, dim3((unsigned)grid), dim3(BIS_THREADS), 0, s, t2, N, nkeep, hi0, pivmin,
                        sigma);
 }

@@ -28,7 +28,7 @@ def compile_kernels(tilesize=32, tilefactor=1, tiledim2d=False):
 
 
 def _opts(sched=None, tree=None, domain=0, update_mode=0, workers=0, wait_ms=0, single_stream=False,
-          phase_timing=False, reserve_cus=0, graph=False):
+          phase_timing=False, reserve_cus=0, graph=False, reserved_modes=0):
     o = Opts()
     lib().avs_default_opts(ctypes.byref(o))
     if sched is not None:
@@ -43,6 +43,8 @@ def _opts(sched=None, tree=None, domain=0, update_mode=0, workers=0, wait_ms=0, 
     o.reserved[2] = 1 if single_stream else 0
     o.reserved[3] = reserve_cus
     o.reserved[4] = 1 if graph else 0
+    # update modes on the CU-masked stream (bit 0 GEQRT's, 1 TSQRT's, 2 merges'; 0 = all)
+    o.reserved[5] = reserved_modes
     return o
 
 

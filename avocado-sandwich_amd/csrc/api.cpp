@@ -337,14 +337,22 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
     // done[nph]: the latest update (or the matrix upload) the next sweep's panels wait for
     hipEvent_t last_update = two ? h->done[nph] : nullptr;
     if (two) HIPCHK(hipEventRecord(last_update, us));
+    // with reserved CUs, the update modes in bit mask opts.reserved[5] (bit m:
+    // mode m; 0 = all) run on the CU-masked stream, the others on the
+    // handle's stream, each after the previous update
+    const int rmask = h->opts.reserved[5] ? h->opts.reserved[5] : 7;
+    hipStream_t prev_us = us;
     int last_panel = -1;
     for (size_t i = 0; i < nph; ++i) {
         const Phase &ph = D.P.phases[i];
-        hipStream_t st = ph.kind == TK_PANEL ? ps : us;
+        hipStream_t ust = (us != h->stream && !((rmask >> ph.mode) & 1)) ? h->stream : us;
+        hipStream_t st = ph.kind == TK_PANEL ? ps : ust;
         if (two) {
             if (ph.kind == TK_PANEL && ph.mode == MODE_GE) HIPCHK(hipStreamWaitEvent(ps, last_update, 0));
-            if (ph.kind == TK_UPDATE && last_panel >= 0) HIPCHK(hipStreamWaitEvent(us, h->done[last_panel], 0));
+            if (ph.kind == TK_UPDATE && ust != prev_us) HIPCHK(hipStreamWaitEvent(ust, last_update, 0));
+            if (ph.kind == TK_UPDATE && last_panel >= 0) HIPCHK(hipStreamWaitEvent(ust, h->done[last_panel], 0));
         }
+        if (ph.kind == TK_UPDATE) prev_us = ust;
         if (timing) {
             h->pkind[i] = ph.kind;
             h->pmode[i] = ph.mode;
@@ -363,7 +371,7 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
         if (timing) HIPCHK(hipEventRecord(h->pev[2 * i + 1], st));
         if (two) {
             if (ph.kind == TK_PANEL) HIPCHK(hipEventRecord(h->done[i], ps));
-            else HIPCHK(hipEventRecord(last_update, us));
+            else HIPCHK(hipEventRecord(last_update, ust));
         }
         ++h->plaunches;
     }

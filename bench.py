@@ -74,6 +74,10 @@ def main():
     ap.add_argument("--sched", choices=["streams", "host"], default="streams")
     ap.add_argument("--tree", choices=["hier", "flat"], default="hier")
     ap.add_argument("--domain", type=int, default=0)
+    ap.add_argument("--reserve-cus", type=int, default=16,
+                    help="CUs (the same number per XCD) the CU-masked update stream leaves to the panels")
+    ap.add_argument("--masked-modes", type=int, default=6,
+                    help="update modes on the CU-masked stream: bit 0 GEQRT's, 1 TSQRT's, 2 merges' (0 = all)")
     ap.add_argument("--cpu-n", type=int, default=1536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check-n", type=int, default=4096,
@@ -114,6 +118,12 @@ def main():
     o.domain = args.domain
     o.reserved[0] = 120000
     o.reserved[1] = 1 if args.sched == "streams" else 0  # per-launch events for the roofline
+    if args.sched == "streams":
+        # the TSQRT and merge updates on a stream that leaves reserve_cus CUs to
+        # the panels (n=16384 FP64 b=128: 459-460 ms against 467-469 ms with
+        # every update on all CUs; bit-identical bands)
+        o.reserved[3] = args.reserve_cus
+        o.reserved[5] = args.masked_modes
     h = ctypes.c_void_p()
     check(L.avs_create(ctypes.byref(h), code, n, b, ctypes.byref(o)))
     pdA = ctypes.c_void_p(dA.data_ptr())
@@ -229,6 +239,8 @@ def main():
         "config": {"workload": "Blockbidiag! n=%d %s tile b=%d, Gaussian input resident in HBM" %
                                (n, args.dtype.upper(), b),
                    "n": n, "b": b, "sched": args.sched, "tree": args.tree, "domain": args.domain or "auto",
+                   "reserved_cus": args.reserve_cus if args.sched == "streams" else 0,
+                   "masked_update_modes": args.masked_modes if args.sched == "streams" else 0,
                    "parallelism": "single GPU"},
         "pct_of_peak": round(100.0 * value / peak, 2),
         "seconds_per_reduction": round(ms_per_step / 1e3, 4),

@@ -84,6 +84,10 @@ typedef struct avs_opts {
                               reserved[3] = K > 0: AVS_SCHED_STREAMS runs the updates
                               on a stream whose CU mask leaves K CUs (the same number
                               on every XCD; K <= CUs/8 - CUs/32) to the panels.
+                              reserved[5] (with reserved[3] > 0): bit mask of the
+                              update modes (1 GEQRT's, 2 TSQRT's, 4 the tree
+                              merges') that run on the CU-masked stream; the
+                              others run on the handle's stream; 0 = all.
                               reserved[4] != 0: AVS_SCHED_STREAMS captures the
                               first run of a handle's phase program as a HIP
                               graph and replays it on every run (not with

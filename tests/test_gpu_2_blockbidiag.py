@@ -121,17 +121,20 @@ def test_panel_stream_overlap_is_bitwise_neutral(gpu, n, b, dt, tree):
 
 
 def test_reserved_cus_are_bitwise_neutral(gpu):
-    """Updates on a CU-masked stream (16 CUs left to the panels) give the same
-    bits as the default placement."""
+    """Updates on a CU-masked stream (16 CUs left to the panels), all of them
+    or only the tree merges' and TSQRT's, give the same bits as the default
+    placement."""
     n, b = 1024, 128
     a = np.asfortranarray(O.gaussian_matrix(n, BASE + 29))
     outs = []
-    for k in (0, 16):
-        print("case n=%d b=%d reserve_cus=%d" % (n, b, k), flush=True)
-        T = avsvd.TiledMatrix(np.asfortranarray(a.copy()), b, b, sched=avsvd.SCHED_STREAMS, reserve_cus=k)
+    for k, modes in ((0, 0), (16, 0), (16, 4), (16, 6)):
+        print("case n=%d b=%d reserve_cus=%d modes=%d" % (n, b, k, modes), flush=True)
+        T = avsvd.TiledMatrix(np.asfortranarray(a.copy()), b, b, sched=avsvd.SCHED_STREAMS, reserve_cus=k,
+                              reserved_modes=modes)
         avsvd.Blockbidiag_(T)
         outs.append(T.A.copy())
-    np.testing.assert_array_equal(outs[0], outs[1])
+    for o in outs[1:]:
+        np.testing.assert_array_equal(outs[0], o)
 
 
 @pytest.mark.parametrize("sched,tree", [(avsvd.SCHED_HOST, avsvd.TREE_FLAT),

@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--domain", type=int, default=0)
     ap.add_argument("--reserve-cus", type=int, default=16,
                     help="CUs (the same number per XCD) the CU-masked update stream leaves to the panels")
-    ap.add_argument("--masked-modes", type=int, default=6,
+    ap.add_argument("--masked-modes", type=int, default=2,
                     help="update modes on the CU-masked stream: bit 0 GEQRT's, 1 TSQRT's, 2 merges' (0 = all)")
     ap.add_argument("--cpu-n", type=int, default=1536)
     ap.add_argument("--no-cpu", action="store_true")
@@ -119,9 +119,11 @@ def main():
     o.reserved[0] = 120000
     o.reserved[1] = 1 if args.sched == "streams" else 0  # per-launch events for the roofline
     if args.sched == "streams":
-        # the TSQRT and merge updates on a stream that leaves reserve_cus CUs to
-        # the panels (n=16384 FP64 b=128: 459-460 ms against 467-469 ms with
-        # every update on all CUs; bit-identical bands)
+        # the TSQRT updates on a stream that leaves reserve_cus CUs to the
+        # panels, so that the first merge panels of a sweep start at once
+        # (n=16384 FP64 b=128: 452-453 ms against 467-469 ms with every update
+        # on all CUs and 459-460 ms with the merge updates masked too;
+        # bit-identical bands)
         o.reserved[3] = args.reserve_cus
         o.reserved[5] = args.masked_modes
     h = ctypes.c_void_p()

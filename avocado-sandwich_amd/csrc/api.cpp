@@ -550,10 +550,10 @@ avs_status chase_on_device(avs_handle *h, double *d, double *e)
 // Where avs_svdvals runs the band -> bidiagonal stage: AVSVD_STAGE2=device on
 // the device; any other value (host, halving, dgbbrd) on the host; unset, on
 // the device from n = AVSVD_DEVICE_CHASE_MIN_N (default 16384) up.  Each time
-// of the device schedule is one launch of about 15 us (b = 32) to 50 us
+// of the device schedule is one launch of about 15 us (b = 32) to 45 us
 // (b = 128) on MI355X, and there are about 3n of them, so below n = 16384 the
-// host chase on 16 threads is faster (n = 16384: 0.97 vs 0.98 s at b = 32,
-// 1.29 vs 1.53 s at b = 64, 2.57 vs 2.59 s at b = 128).
+// host chase on 16 threads is faster (n = 16384, after the reduction: 1.03 vs
+// 1.10 s at b = 32, 1.41 vs 1.70 s at b = 64, 2.31 vs 2.80 s at b = 128).
 bool svdvals_chase_on_device(int64_t n, int b)
 {
     if (b < 2 || b > 128) return false;

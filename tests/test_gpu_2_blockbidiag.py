@@ -163,14 +163,14 @@ def test_kend_partial_matches_oracle(gpu, sched):
     np.testing.assert_allclose(T.A, ref, atol=1e-11 * np.sqrt(n))
 
 
-def test_config1_svdvals_end_to_end(gpu, golden_dir):
+def test_n1024_svdvals_end_to_end(gpu, golden_dir):
     """The CPU plain-path workload (n=1024, b=64, FP64, seed index 1) on the device path."""
     n, b = 1024, 64
     a = np.asfortranarray(O.gaussian_matrix(n, BASE + 1))
-    ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["config1_n1024"]
+    ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["gauss_n1024_seed1"]
     sig = avsvd.svdvals(a, tilesize=b)
     err = O.sigma_err(sig, ref)
-    print("config1 sigma err %.3e" % err, flush=True)
+    print("n1024 sigma err %.3e" % err, flush=True)
     assert err <= 10 * n * np.finfo(np.float64).eps
 
 

@@ -141,11 +141,11 @@ def test_bidiagonalize_preserves_sigma(n, bw):
     assert np.abs(off).max() < 1e-12 * np.abs(a).max()
 
 
-def test_config1_pipeline_against_gesdd(golden_dir):
+def test_n1024_pipeline_against_gesdd(golden_dir):
     # the CPU plain-path workload: n=1024, b=64 Gaussian through band + host stage
     n, b = 1024, 64
     a = O.gaussian_matrix(n, BASE + 1)
-    ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["config1_n1024"]
+    ref = np.load(os.path.join(golden_dir, "sigma_small.npz"))["gauss_n1024_seed1"]
     band = O.blockbidiag(a, b)
     sig = O.svdvals_host_stage(band, b)
     assert O.sigma_err(sig, ref) < 10 * n * np.finfo(float).eps

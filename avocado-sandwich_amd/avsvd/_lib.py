@@ -43,7 +43,9 @@ class ServerOpts(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("stream", ctypes.c_void_p), ("sched", ctypes.c_int32), ("tree", ctypes.c_int32),
                 ("domain", ctypes.c_int32), ("update_mode", ctypes.c_int32), ("workers", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 9)]
+                ("wait_ms", ctypes.c_int32), ("phase_timing", ctypes.c_int32), ("single_stream", ctypes.c_int32),
+                ("reserve_cus", ctypes.c_int32), ("graph", ctypes.c_int32), ("masked_modes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class AvsError(RuntimeError):

@@ -28,7 +28,7 @@ def compile_kernels(tilesize=32, tilefactor=1, tiledim2d=False):
 
 
 def _opts(sched=None, tree=None, domain=0, update_mode=0, workers=0, wait_ms=0, single_stream=False,
-          phase_timing=False, reserve_cus=0, graph=False, reserved_modes=0):
+          phase_timing=False, reserve_cus=0, graph=False, masked_modes=0):
     o = Opts()
     lib().avs_default_opts(ctypes.byref(o))
     if sched is not None:
@@ -38,13 +38,13 @@ def _opts(sched=None, tree=None, domain=0, update_mode=0, workers=0, wait_ms=0, 
     o.domain = domain
     o.update_mode = update_mode
     o.workers = workers
-    o.reserved[0] = wait_ms
-    o.reserved[1] = 1 if phase_timing else 0
-    o.reserved[2] = 1 if single_stream else 0
-    o.reserved[3] = reserve_cus
-    o.reserved[4] = 1 if graph else 0
+    o.wait_ms = wait_ms
+    o.phase_timing = 1 if phase_timing else 0
+    o.single_stream = 1 if single_stream else 0
+    o.reserve_cus = reserve_cus
+    o.graph = 1 if graph else 0
     # update modes on the CU-masked stream (bit 0 GEQRT's, 1 TSQRT's, 2 merges'; 0 = all)
-    o.reserved[5] = reserved_modes
+    o.masked_modes = masked_modes
     return o
 
 

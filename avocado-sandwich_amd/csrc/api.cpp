@@ -131,7 +131,7 @@ struct avs_handle {
     long long wallclock_khz = 100000;
     std::shared_ptr<DevProgram> prog;
     std::shared_ptr<DevPhases> phases;
-    // per-launch timing of the stream-ordered executor (opts.reserved[1] != 0):
+    // per-launch timing of the stream-ordered executor (opts.phase_timing != 0):
     // pev[i] .. pev[i+1] brackets launch i, whose kind is pkind[i]
     std::vector<hipEvent_t> pev;
     std::vector<int> pkind, pmode;
@@ -145,7 +145,7 @@ struct avs_handle {
     // phase (the panels of a sweep run ahead of that sweep's updates)
     hipStream_t pstream = nullptr;
     std::vector<hipEvent_t> done;
-    // update stream whose CU mask leaves opts.reserved[3] CUs to the panels,
+    // update stream whose CU mask leaves opts.reserve_cus CUs to the panels,
     // and the events that join it to the handle's stream
     hipStream_t ustream = nullptr;
     hipEvent_t ujoin[2] = {nullptr, nullptr};
@@ -154,14 +154,11 @@ struct avs_handle {
 namespace {
 
 int resolve_tree(const avs_handle *h) { return h->opts.tree; }
-int resolve_domain(const avs_handle *h)
-{
-    if (h->opts.domain > 0) return h->opts.domain;
-    // Two tiles per domain: the panel chain of a sweep is then 2 + log2(m/2)
-    // panels long (the shortest for m >= 4 up to one), and the update work is
-    // the same for every domain size.
-    return 2;
-}
+// 0 = the automatic per-sweep rule of domain_for (two tiles per domain by
+// default: the panel chain of a sweep is then 2 + log2(m/2) panels long, the
+// shortest for m >= 4 up to one, and the update work is the same for every
+// domain size)
+int resolve_domain(const avs_handle *h) { return h->opts.domain > 0 ? h->opts.domain : 0; }
 int upd_tiles_for(int b) { return b >= 128 ? 1 : (b == 64 ? 2 : 4); }
 
 avs_status ensure_program(avs_handle *h, int alg, int kend)
@@ -275,18 +272,18 @@ double phase_update_flops(const PhaseProgram &P, const Phase &ph)
 }
 
 // Stream-ordered execution.  Updates run in program order on the handle's
-// stream.  Unless opts.reserved[2] is set, panels run in program order on a
+// stream.  Unless opts.single_stream is set, panels run in program order on a
 // second stream: within a sweep the panel chain (GEQRT, TSQRT chains, merge
 // levels) does not depend on that sweep's updates, so it runs ahead of them;
 // each update phase waits for its own panel phase, and the first panel phase
-// of a sweep waits for every update before it.  With opts.reserved[1] set,
+// of a sweep waits for every update before it.  With opts.phase_timing set,
 // every launch is bracketed by timing events on its stream.
 template <typename TS, int B> avs_status run_streams(avs_handle *h)
 {
     const DevPhases &D = *h->phases;
     const int per = upd_tiles_for(B);
-    const bool timing = h->opts.reserved[1] != 0;
-    const bool two = h->opts.reserved[2] == 0;
+    const bool timing = h->opts.phase_timing != 0;
+    const bool two = h->opts.single_stream == 0;
     const size_t nph = D.P.phases.size();
     h->plaunches = 0;
     if (timing) {
@@ -308,7 +305,7 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
         }
     }
     hipStream_t us = h->stream, ps = two ? h->pstream : h->stream;
-    const int reserve = two ? h->opts.reserved[3] : 0;
+    const int reserve = two ? h->opts.reserve_cus : 0;
     if (reserve > 0) {
         if (!h->ustream) {
             int ncu = 0;
@@ -337,10 +334,10 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
     // done[nph]: the latest update (or the matrix upload) the next sweep's panels wait for
     hipEvent_t last_update = two ? h->done[nph] : nullptr;
     if (two) HIPCHK(hipEventRecord(last_update, us));
-    // with reserved CUs, the update modes in bit mask opts.reserved[5] (bit m:
+    // with reserved CUs, the update modes in bit mask opts.masked_modes (bit m:
     // mode m; 0 = all) run on the CU-masked stream, the others on the
     // handle's stream, each after the previous update
-    const int rmask = h->opts.reserved[5] ? h->opts.reserved[5] : 7;
+    const int rmask = h->opts.masked_modes ? h->opts.masked_modes : 7;
     hipStream_t prev_us = us;
     int last_panel = -1;
     for (size_t i = 0; i < nph; ++i) {
@@ -390,15 +387,15 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
 // capture through the events it waits on, and rejoins before the capture
 // ends; the second stream and the events are created before the capture) and
 // every run replays it.  The graph has kernel and event nodes only, no
-// memset.  Only with reserved[4] != 0 and without per-launch timing
-// (reserved[1]), one stream (reserved[2]) or the CU-masked update stream
-// (reserved[3]): replaying the n=16384, b=128 FP64 graph (3571 kernels) took
+// memset.  Only with opts.graph != 0 and without per-launch timing
+// (phase_timing), one stream (single_stream) or the CU-masked update stream
+// (reserve_cus): replaying the n=16384, b=128 FP64 graph (3571 kernels) took
 // 527-544 ms against 460-464 ms for the eager enqueue on the same MI355X, so
 // eager is the default.
 template <typename TS, int B> avs_status run_streams_graph(avs_handle *h)
 {
-    const bool graph = h->opts.reserved[1] == 0 && h->opts.reserved[2] == 0 && h->opts.reserved[3] == 0 &&
-                       h->opts.reserved[4] != 0;
+    const bool graph = h->opts.phase_timing == 0 && h->opts.single_stream == 0 && h->opts.reserve_cus == 0 &&
+                       h->opts.graph != 0;
     if (!graph) return run_streams<TS, B>(h);
     if (h->gphases != h->phases) {
         if (h->gexec) HIPCHK(hipGraphExecDestroy(h->gexec));
@@ -569,8 +566,8 @@ template <typename TS, int B> avs_status run_persistent(avs_handle *h)
     HIPCHK(hipMemsetAsync(h->dCnt, 0, sizeof(int) * (size_t)h->nt * h->nt, h->stream));
     HIPCHK(hipMemsetAsync(h->dTicket, 0, sizeof(int), h->stream));
     HIPCHK(hipMemsetAsync(h->dStatus, 0, sizeof(int), h->stream));
-    // dependency wait bound: opts.reserved[0] milliseconds (default 30 s)
-    const long long ms = h->opts.reserved[0] > 0 ? h->opts.reserved[0] : 30000;
+    // dependency wait bound: opts.wait_ms milliseconds (default 30 s)
+    const long long ms = h->opts.wait_ms > 0 ? h->opts.wait_ms : 30000;
     launch_persistent<TS, B>(h->workers, h->dTiles, h->dTau, h->dTjj, h->prog->dTasks, (int)P.order.size(),
                              h->prog->dDepTile, h->prog->dDepVal, h->prog->dSig, h->dCnt, h->dTicket,
                              ms * h->wallclock_khz, h->dStatus, split_of(h), h->stream);
@@ -639,6 +636,9 @@ avs_status avs_create(avs_handle **hp, int32_t dtype, int64_t n, int32_t b, cons
     if (n <= 0 || b <= 0 || n % b != 0) return fail(-3, "Not supported: n=%lld is not a positive multiple of b=%d",
                                                     (long long)n, b);
     if (!supported_b(b)) return fail(-4, "Not supported: tile size %d (built: 32, 64, 128)", b);
+    if (opts)
+        for (int32_t r : opts->reserved)
+            if (r != 0) return fail(-5, "opts.reserved must be zero");
     std::unique_ptr<avs_handle> h(new avs_handle());
     h->dtype = dtype;
     h->n = n;
@@ -830,8 +830,8 @@ avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_laun
                             int64_t *update_launches, double *update_flops)
 {
     if (!h) return fail(-1, "null handle");
-    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.reserved[1] == 0 || h->plaunches == 0)
-        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.reserved[1] = 1)");
+    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.phase_timing == 0 || h->plaunches == 0)
+        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.phase_timing = 1)");
     HIPCHK(hipStreamSynchronize(h->stream));
     double pm = 0, um = 0, uf = 0;
     int64_t pn = 0, un = 0;
@@ -853,8 +853,8 @@ avs_status avs_phase_trace(avs_handle *h, double *start_ms, double *end_ms, int3
                            int64_t *count)
 {
     if (!h) return fail(-1, "null handle");
-    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.reserved[1] == 0 || h->plaunches == 0)
-        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.reserved[1] = 1)");
+    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.phase_timing == 0 || h->plaunches == 0)
+        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.phase_timing = 1)");
     if (count) *count = h->plaunches;
     if (!start_ms || !end_ms || !kind_mode) return AVS_OK;
     if (cap < h->plaunches) return fail(-5, "cap < number of launches");

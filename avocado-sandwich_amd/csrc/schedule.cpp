@@ -21,11 +21,45 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <functional>
 #include <queue>
+#include <string>
 #include <utility>
 
 namespace avs {
+
+int domain_for(int m, int domain)
+{
+    if (domain > 0) return domain;
+    // (m_i, d_i) sorted by m_i; parsed once (a function-local static is
+    // initialised thread-safely)
+    static const std::vector<std::pair<int, int>> rule = [] {
+        std::vector<std::pair<int, int>> r;
+        if (const char *e = getenv("AVSVD_DOMAIN_RULE")) {
+            std::string s(e);
+            size_t pos = 0;
+            while (pos < s.size()) {
+                size_t end = s.find(',', pos);
+                if (end == std::string::npos) end = s.size();
+                const std::string item = s.substr(pos, end - pos);
+                const size_t c = item.find(':');
+                if (c != std::string::npos) {
+                    const int m0 = atoi(item.substr(0, c).c_str()), d0 = atoi(item.substr(c + 1).c_str());
+                    if (m0 >= 0 && d0 >= 1) r.push_back({m0, d0});
+                }
+                pos = end + 1;
+            }
+        }
+        if (r.empty()) r.push_back({0, 2});
+        std::sort(r.begin(), r.end());
+        return r;
+    }();
+    int d = rule.front().second;
+    for (const auto &x : rule)
+        if (m >= x.first) d = x.second;
+    return d;
+}
 
 namespace {
 
@@ -108,7 +142,7 @@ struct Builder {
         const int cnt = nt - k - 1;
         const int nt2 = nt * nt;
         const int famGE = lq ? 2 : 0, famTT = lq ? 3 : 1;
-        const int h = (tree == 0 || domain <= 0) ? m : domain;
+        const int h = tree == 0 ? m : domain_for(m, domain);
         std::vector<int> heads;
         for (int d0 = 0; d0 < m; d0 += h) {
             heads.push_back(d0);
@@ -301,7 +335,7 @@ struct PhaseBuilder {
         const int cnt = nt - k - 1;
         const int nt2 = nt * nt;
         const int famGE = lq ? 2 : 0, famTT = lq ? 3 : 1;
-        const int h = (tree == 0 || domain <= 0) ? m : domain;
+        const int h = tree == 0 ? m : domain_for(m, domain);
         std::vector<int> heads;
         for (int d0 = 0; d0 < m; d0 += h) heads.push_back(d0);
         // GEQRT of every domain head, then its update

@@ -29,7 +29,13 @@ struct AvsOpts
     domain::Int32
     update_mode::Int32
     workers::Int32
-    reserved::NTuple{9,Int32}
+    wait_ms::Int32
+    phase_timing::Int32
+    single_stream::Int32
+    reserve_cus::Int32
+    graph::Int32
+    masked_modes::Int32
+    reserved::NTuple{3,Int32}
 end
 
 function default_opts()

@@ -52,7 +52,7 @@ def panel_rate(n, b, tree, domain, dtype, panel_ms):
     import avsvd
     elt = 8 if dtype == "f64" else 4
     ops = {0: 0, 1: 0, 2: 0}
-    for kind, _, chains in avsvd.phase_dump(n // b, b, 0, 0, tree, domain or 2):
+    for kind, _, chains in avsvd.phase_dump(n // b, b, 0, 0, tree, domain):
         if kind == 0:
             for ch in chains:
                 for op in ch:
@@ -116,16 +116,16 @@ def main():
     o.sched = avsvd.SCHED_STREAMS if args.sched == "streams" else avsvd.SCHED_HOST
     o.tree = avsvd.TREE_HIER if args.tree == "hier" else avsvd.TREE_FLAT
     o.domain = args.domain
-    o.reserved[0] = 120000
-    o.reserved[1] = 1 if args.sched == "streams" else 0  # per-launch events for the roofline
+    o.wait_ms = 120000
+    o.phase_timing = 1 if args.sched == "streams" else 0  # per-launch events for the roofline
     if args.sched == "streams":
         # the TSQRT updates on a stream that leaves reserve_cus CUs to the
         # panels, so that the first merge panels of a sweep start at once
         # (n=16384 FP64 b=128: 452-453 ms against 467-469 ms with every update
         # on all CUs and 459-460 ms with the merge updates masked too;
         # bit-identical bands)
-        o.reserved[3] = args.reserve_cus
-        o.reserved[5] = args.masked_modes
+        o.reserve_cus = args.reserve_cus
+        o.masked_modes = args.masked_modes
     h = ctypes.c_void_p()
     check(L.avs_create(ctypes.byref(h), code, n, b, ctypes.byref(o)))
     pdA = ctypes.c_void_p(dA.data_ptr())

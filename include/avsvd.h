@@ -70,30 +70,28 @@ typedef struct avs_opts {
     void *stream;          /* hipStream_t; 0 = create one */
     int32_t sched;         /* avs_sched */
     int32_t tree;          /* avs_tree */
-    int32_t domain;        /* tiles per domain for AVS_TREE_HIER (0 = auto) */
+    int32_t domain;        /* tiles per domain for AVS_TREE_HIER (0 = automatic per-sweep rule) */
     int32_t update_mode;   /* avs_update_mode */
     int32_t workers;       /* persistent workgroups (0 = one per CU) */
-    int32_t reserved[9];   /* reserved[0]: bound in ms on any one dependency wait of
-                              the persistent executor (0 = 30000); exceeding it ends
-                              the launch and the next sync returns AVS_ERR_STATE.
-                              reserved[1] != 0: per-launch timing events for
-                              avs_phase_timing (AVS_SCHED_STREAMS).
-                              reserved[2] != 0: AVS_SCHED_STREAMS runs panels on the
-                              handle's stream too (by default they run on a second
-                              stream of the handle, ahead of the same sweep's updates).
-                              reserved[3] = K > 0: AVS_SCHED_STREAMS runs the updates
-                              on a stream whose CU mask leaves K CUs (the same number
-                              on every XCD; K <= CUs/8 - CUs/32) to the panels.
-                              reserved[5] (with reserved[3] > 0): bit mask of the
-                              update modes (1 GEQRT's, 2 TSQRT's, 4 the tree
-                              merges') that run on the CU-masked stream; the
-                              others run on the handle's stream; 0 = all.
-                              reserved[4] != 0: AVS_SCHED_STREAMS captures the
-                              first run of a handle's phase program as a HIP
-                              graph and replays it on every run (not with
-                              reserved[1..3]); by default every launch is
-                              enqueued each run (faster on MI355X).
-                              The rest must be 0. */
+    int32_t wait_ms;       /* bound in ms on any one dependency wait of the persistent
+                              executor (0 = 30000); exceeding it ends the launch and the
+                              next sync returns AVS_ERR_STATE */
+    int32_t phase_timing;  /* != 0: AVS_SCHED_STREAMS brackets every launch with timing
+                              events for avs_phase_timing / avs_phase_trace */
+    int32_t single_stream; /* != 0: AVS_SCHED_STREAMS runs the panels on the handle's
+                              stream too (by default they run on a second stream of the
+                              handle, ahead of the same sweep's updates) */
+    int32_t reserve_cus;   /* K > 0: AVS_SCHED_STREAMS runs the updates on a stream whose
+                              CU mask leaves K CUs (the same number on every XCD;
+                              K <= CUs/8 - CUs/32) to the panels */
+    int32_t graph;         /* != 0: AVS_SCHED_STREAMS captures the first run of a handle's
+                              phase program as a HIP graph and replays it on every run
+                              (not with phase_timing, single_stream or reserve_cus); by
+                              default every launch is enqueued each run (faster on MI355X) */
+    int32_t masked_modes;  /* with reserve_cus > 0: bit mask of the update modes (1 GEQRT's,
+                              2 TSQRT's, 4 the tree merges') that run on the CU-masked
+                              stream; the others run on the handle's stream; 0 = all */
+    int32_t reserved[3];   /* must be 0 (avs_create returns -5 otherwise) */
 } avs_opts;
 
 typedef struct avs_handle avs_handle;
@@ -104,7 +102,7 @@ void avs_default_opts(avs_opts *opts);
 /* TiledMatrix(A, b, b) for an n x n matrix: allocates the tile-major working
  * copy, tau/T factor storage and dependency counters.  Errors like the
  * reference's "Not supported" (TiledMatrix.jl:42-44): n % b != 0 -> -3,
- * unsupported b -> -4. */
+ * unsupported b -> -4; nonzero opts->reserved -> -5. */
 avs_status avs_create(avs_handle **h, int32_t dtype, int64_t n, int32_t b, const avs_opts *opts);
 avs_status avs_destroy(avs_handle *h);
 
@@ -130,7 +128,7 @@ avs_status avs_blockbidiag(avs_handle *h, int32_t kend);
 avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
 
 /* Per-launch device time of the last avs_blockbidiag / avs_tiled_qr run under
- * AVS_SCHED_STREAMS, for handles created with opts.reserved[1] != 0 (one event
+ * AVS_SCHED_STREAMS, for handles created with opts.phase_timing != 0 (one event
  * pair per launch, on the stream the launch runs on; with the default second
  * panel stream, panel and update launches overlap in time, so the two sums can
  * exceed the run's wall time): summed milliseconds and launch counts of the panel and
@@ -139,7 +137,7 @@ avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
  * the run.  No counterpart in the reference (its benchmarks time whole calls). */
 avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_launches, double *update_ms,
                             int64_t *update_launches, double *update_flops);
-/* Per-launch timeline of the last timed stream-ordered run (opts.reserved[1]):
+/* Per-launch timeline of the last timed stream-ordered run (opts.phase_timing):
  * for launch i, start and end in ms from the first launch's start event, and
  * kind*4 + mode (kind 0 panel, 1 update; mode 0 GE, 1 TS, 2 TT).  With null
  * arrays only *count is set.  No counterpart in the reference. */

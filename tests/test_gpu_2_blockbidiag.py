@@ -130,7 +130,7 @@ def test_reserved_cus_are_bitwise_neutral(gpu):
     for k, modes in ((0, 0), (16, 0), (16, 4), (16, 6)):
         print("case n=%d b=%d reserve_cus=%d modes=%d" % (n, b, k, modes), flush=True)
         T = avsvd.TiledMatrix(np.asfortranarray(a.copy()), b, b, sched=avsvd.SCHED_STREAMS, reserve_cus=k,
-                              reserved_modes=modes)
+                              masked_modes=modes)
         avsvd.Blockbidiag_(T)
         outs.append(T.A.copy())
     for o in outs[1:]:

@@ -37,7 +37,7 @@ EXPORTS = (
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
                 ("host_threads", ctypes.c_int32), ("b_f32", ctypes.c_int32), ("b_f64", ctypes.c_int32),
-                ("update_mode", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+                ("update_mode", ctypes.c_int32), ("device_sigma", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
 
 
 class Opts(ctypes.Structure):

@@ -16,9 +16,10 @@ from ._lib import Opts, check, dptr, dtype_code, lib, vptr
 
 class Batch:
     """Matrices of orders `ns` reduced together; b=None takes the tile size
-    from the autotuner's table (form "batch", keyed by the largest order)."""
+    from the autotuner's table (form "batch", keyed by the largest order).
+    update_mode: UPD_SPLIT_BF16 runs FP32 trailing updates as split-BF16."""
 
-    def __init__(self, ns, dtype=np.float32, b=None, workers=0):
+    def __init__(self, ns, dtype=np.float32, b=None, workers=0, update_mode=0):
         self.ns = [int(x) for x in ns]
         self.dtype = np.dtype(dtype)
         if b is None:
@@ -29,6 +30,7 @@ class Batch:
         o = Opts()
         lib().avs_default_opts(ctypes.byref(o))
         o.workers = workers
+        o.update_mode = update_mode
         arr = (ctypes.c_int64 * len(self.ns))(*self.ns)
         check(lib().avs_batch_create(ctypes.byref(self._h), dtype_code(self.dtype), len(self.ns), arr, b,
                                      ctypes.byref(o)))

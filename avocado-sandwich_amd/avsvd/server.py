@@ -24,10 +24,12 @@ def served_tile(dtype, n):
 class Server:
     """Serving loop.  b_f32 / b_f64 fix the tile size per dtype; 0 leaves it to
     the autotuner's table (`tune_table`: a path, default the table
-    avsvd.autotune.load_table() finds), which is loaded into the library here."""
+    avsvd.autotune.load_table() finds), which is loaded into the library here.
+    device_sigma: the bidiagonals' singular values by bisection on the device
+    (per host worker, buffers allocated once) instead of LAPACK dbdsdc."""
 
     def __init__(self, hbm_budget_gb=64, max_inflight=4, host_threads=0, b_f32=0, b_f64=0, update_mode=0,
-                 tune_table=None):
+                 tune_table=None, device_sigma=False):
         if not (b_f32 and b_f64):
             from .autotune import load_table, push_to_library
             push_to_library(load_table(tune_table))
@@ -38,6 +40,7 @@ class Server:
         if host_threads:
             o.host_threads = host_threads
         o.b_f32, o.b_f64, o.update_mode = b_f32, b_f64, update_mode
+        o.device_sigma = 1 if device_sigma else 0
         self._h = ctypes.c_void_p()
         check(lib().avs_server_create(ctypes.byref(self._h), ctypes.byref(o)))
         self._n = {}

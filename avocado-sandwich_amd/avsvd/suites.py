@@ -210,18 +210,21 @@ def _lapack_sigmas(mats):
     return out
 
 
-def suite_batched(args):
+def batched_run(count=4096, b=0, plain_sample=128, lapack_sample=0, seed=SEED):
+    """The batched form: `count` FP32 matrices, n log-uniform in {64 .. 1024},
+    entries N(0, 1/n), in one grouped launch per sweep; then the plain
+    per-matrix FP64 GPU path, one matrix at a time, on a sample of the same
+    mix.  sigma errors against FP64 LAPACK per matrix (every matrix, or a
+    random sample of `lapack_sample`).  Returns one dict."""
     from .autotune import choose_tile
     from .batch import Batch
     from .tiled import TiledMatrix
     t0 = time.perf_counter()
-    ns, mats = _batched_inputs(args.count, SEED)
+    ns, mats = _batched_inputs(count, seed)
     gen_s = time.perf_counter() - t0
     flops = sum(8.0 / 3.0 * n ** 3 for n in ns)
-    emit({"suite": "batched", "stage": "inputs", "count": len(ns), "gen_s": round(gen_s, 2),
-          "sizes": {str(k): ns.count(k) for k in sorted(set(ns))}, "algorithmic_flops": flops})
-    # batched FP32 form
-    bt = Batch(ns, dtype=np.float32, b=args.b or None)
+    progress("batched: %d inputs in %.1f s" % (len(ns), gen_s))
+    bt = Batch(ns, dtype=np.float32, b=b or None)
     bsel = bt.b
     for i, m in enumerate(mats):
         bt.set(i, m)
@@ -230,130 +233,163 @@ def suite_batched(args):
     total_s = time.perf_counter() - t0
     red_ms = bt.last_time_ms()
     del bt
+    rng = np.random.default_rng(seed + 7)
+    chk = (list(range(len(ns))) if not lapack_sample or lapack_sample >= len(ns)
+           else sorted(rng.choice(len(ns), size=lapack_sample, replace=False).tolist()))
     t0 = time.perf_counter()
-    ref = _lapack_sigmas(mats)
+    ref_list = _lapack_sigmas([mats[i] for i in chk])
     lap_s = time.perf_counter() - t0
-    errs = [sigma_err(s, r) for s, r in zip(sig, ref)]
-    emit({"suite": "batched", "form": "fp32 grouped launch", "count": len(ns), "b": bsel,
-          "band_reduction_s": round(red_ms / 1e3, 4), "band_tflops": round(flops / red_ms / 1e9, 3),
-          "svdvals_total_s": round(total_s, 3), "svdvals_per_s": round(len(ns) / total_s, 2),
-          "max_sigma_err_over_sigma_max": max(errs), "bar": 1e-5,
-          "reference": "FP64 LAPACK svdvals per matrix (%.1f s on %d host threads)" % (lap_s, 16)})
-    # plain per-matrix FP64 GPU path, one matrix at a time, on a sample of the same mix
-    rng = np.random.default_rng(SEED + 7)
-    pick = rng.choice(len(ns), size=min(args.plain_sample, len(ns)), replace=False)
+    ref = dict(zip(chk, ref_list))
+    errs = [sigma_err(sig[i], ref[i]) for i in chk]
+    pick = rng.choice(len(ns), size=min(plain_sample, len(ns)), replace=False)
     t0 = time.perf_counter()
     perrs = []
     for c, i in enumerate(pick):
         if c % 16 == 0:
             progress("plain path %d/%d" % (c, len(pick)))
         A = np.asfortranarray(mats[i].astype(np.float64))
-        b = choose_tile(ns[i], np.float64, "svdvals")[0]
-        T = TiledMatrix(A, b, b)
-        s = T.svdvals()
-        perrs.append(sigma_err(s, ref[i]))
+        bb = choose_tile(ns[i], np.float64, "svdvals")[0]
+        T = TiledMatrix(A, bb, bb)
+        s2 = T.svdvals()
+        if i in ref:
+            perrs.append(sigma_err(s2, ref[i]))
         del T
     plain_s = time.perf_counter() - t0
+    # the plain path's rate over the batch's own size mix: mean time per
+    # matrix of each size class in the sample, weighted by the batch's counts
     plain_rate = len(pick) / plain_s
-    # the sample's rate scaled by the flop share is biased by the size mix; report both the raw
-    # rate and the rate over the same size mix (time per matrix of each size class)
-    emit({"suite": "batched", "form": "plain per-matrix fp64 path, one at a time", "sample": len(pick),
-          "svdvals_per_s": round(plain_rate, 2), "max_sigma_err_over_sigma_max": max(perrs),
-          "speedup_batched_over_plain": round((len(ns) / total_s) / plain_rate, 2)})
+    return {"count": len(ns), "b": bsel, "sizes": {str(k): ns.count(k) for k in sorted(set(ns))},
+            "algorithmic_flops": flops, "band_reduction_s": round(red_ms / 1e3, 4),
+            "band_tflops": round(flops / red_ms / 1e9, 3), "svdvals_total_s": round(total_s, 3),
+            "svdvals_per_s": round(len(ns) / total_s, 2), "max_sigma_err_over_sigma_max": max(errs), "bar": 1e-5,
+            "checked": len(chk),
+            "reference": "FP64 LAPACK svdvals per matrix (%d matrices, %.1f s on 16 host threads)" % (len(chk), lap_s),
+            "plain_fp64_one_at_a_time": {"sample": len(pick), "svdvals_per_s": round(plain_rate, 2),
+                                         "max_sigma_err_over_sigma_max": max(perrs) if perrs else None},
+            "speedup_over_plain_fp64": round((len(ns) / total_s) / plain_rate, 2)}
 
 
-def suite_serving(args):
+def suite_batched(args):
+    d = batched_run(args.count, args.b, args.plain_sample)
+    d["suite"] = "batched"
+    emit(d)
+
+
+def _serving_inputs(requests, nmax, seed):
     import torch
-    from .autotune import load_table, push_to_library
-    from .server import Server, served_tile
-    from .tiled import TiledMatrix
-    # the plain path below takes the tile size the serving loop gives each
-    # request and the same last step (LAPACK dbdsdc on the host)
-    push_to_library(load_table())
-    os.environ["AVSVD_BDSDC"] = "host"
-    rng = np.random.default_rng(SEED + 11)
+    rng = np.random.default_rng(seed + 11)
     reqs = []
-    for i in range(args.requests):
-        n = int(round(math.exp(rng.uniform(math.log(256), math.log(args.nmax)))))
+    for i in range(requests):
+        n = int(round(math.exp(rng.uniform(math.log(256), math.log(nmax)))))
         dt = np.float32 if rng.random() < 0.7 else np.float64
         kind = "gauss" if rng.random() < 0.5 else "geom"
-        reqs.append((n, dt, kind, SEED + 100 + i))
+        reqs.append((n, dt, kind, seed + 100 + i))
     mats, refs = [], []
-    t0 = time.perf_counter()
-    for c, (n, dt, kind, seed) in enumerate(reqs):
+    for c, (n, dt, kind, sd) in enumerate(reqs):
         progress("input %d/%d n=%d %s %s" % (c, len(reqs), n, np.dtype(dt).name, kind))
         if kind == "gauss":
-            a = gaussian_matrix(n, seed, dtype=np.float64)
+            a = gaussian_matrix(n, sd, dtype=np.float64)
             ref = None
         else:
-            a, s = inputs.geometric_matrix_host(n, seed, 1e4, dtype=np.float64)
-            ref = s
+            a, ref = inputs.geometric_matrix_host(n, sd, 1e4, dtype=np.float64)
         mats.append(np.asfortranarray(a.astype(dt)))
         refs.append(ref)
     torch.cuda.empty_cache()
-    emit({"suite": "serving", "stage": "inputs", "requests": len(reqs), "gen_s": round(time.perf_counter() - t0, 2),
-          "n": [r[0] for r in reqs], "fp32_share": sum(r[1] == np.float32 for r in reqs) / len(reqs)})
-    # one-at-a-time plain GPU path (sequential, same requests)
-    t0 = time.perf_counter()
-    lat_plain = []
-    for c, m in enumerate(mats):
-        progress("one-at-a-time %d/%d (n=%d)" % (c, len(mats), m.shape[0]))
-        s0 = time.perf_counter()
-        n = m.shape[0]
-        b = served_tile(m.dtype, n)
-        N = -(-n // b) * b
-        A = np.zeros((N, N), dtype=m.dtype, order="F")
-        A[:n, :n] = m
-        T = TiledMatrix(A, b, b)
-        T.svdvals()
-        del T
-        lat_plain.append((time.perf_counter() - s0) * 1e3)
-    plain_s = time.perf_counter() - t0
-    plain_rate = len(mats) / plain_s
-    # Poisson arrivals at 80% of the one-at-a-time capacity
-    lam = 0.8 * plain_rate
-    gaps = rng.exponential(1.0 / lam, size=len(mats))
-    srv = Server(hbm_budget_gb=args.budget_gb, max_inflight=args.inflight, host_threads=16)
+    return rng, reqs, mats, refs
+
+
+def _serve(mats, gaps, budget_gb, inflight, device_sigma):
+    """Submits mats at the given inter-arrival gaps (None: all at once) and
+    waits for them; (outputs, wall seconds, server stats)."""
+    from .server import Server
+    srv = Server(hbm_budget_gb=budget_gb, max_inflight=inflight, host_threads=16, device_sigma=device_sigma)
     ids = []
     t0 = time.perf_counter()
     tnext = t0
-    for m, g in zip(mats, gaps):
-        tnext += g
-        while time.perf_counter() < tnext:
-            time.sleep(min(0.002, max(0.0, tnext - time.perf_counter())))
+    for k, m in enumerate(mats):
+        if gaps is not None:
+            tnext += gaps[k]
+            while time.perf_counter() < tnext:
+                time.sleep(min(0.002, max(0.0, tnext - time.perf_counter())))
         ids.append(srv.submit(m))
     outs = []
     for c, i in enumerate(ids):
         outs.append(srv.wait(i))
         progress("served %d/%d" % (c + 1, len(ids)))
-    served_s = time.perf_counter() - t0
+    wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    errs = []
-    for (n, dt, kind, seed), (sig, lat), ref in zip(reqs, outs, refs):
-        if ref is not None:
-            errs.append(sigma_err(sig, ref))
-    # the one-at-a-time path under the same arrivals: a FIFO single server whose
-    # service times are the ones measured above
+    return outs, wall, st
+
+
+def serving_run(requests=48, nmax=16384, budget_gb=64, inflight=4, device_sigma=False, seed=SEED):
+    """The served form: requests with n log-uniform in [256, nmax], 70 %% FP32,
+    50/50 Gaussian / geometric (cond 1e4).  First the one-at-a-time plain GPU
+    path on the same requests (and the same requests again in FP64, the plain
+    per-matrix FP64 path), then the serving loop under Poisson arrivals at
+    80 %% of the one-at-a-time capacity, then at saturation (every request
+    submitted at once).  Returns one dict."""
+    from .autotune import load_table, push_to_library
+    from .server import served_tile
+    from .tiled import TiledMatrix
+    push_to_library(load_table())
+    t0 = time.perf_counter()
+    rng, reqs, mats, refs = _serving_inputs(requests, nmax, seed)
+    gen_s = time.perf_counter() - t0
+
+    def one_at_a_time(ms):
+        lat = []
+        t1 = time.perf_counter()
+        for c, m in enumerate(ms):
+            progress("one-at-a-time %d/%d (n=%d %s)" % (c, len(ms), m.shape[0], m.dtype.name))
+            s0 = time.perf_counter()
+            n = m.shape[0]
+            b = served_tile(m.dtype, n)
+            N = -(-n // b) * b
+            A = np.zeros((N, N), dtype=m.dtype, order="F")
+            A[:n, :n] = m
+            T = TiledMatrix(A, b, b)
+            T.svdvals()
+            del T
+            lat.append((time.perf_counter() - s0) * 1e3)
+        return lat, len(ms) / (time.perf_counter() - t1)
+
+    lat_plain, plain_rate = one_at_a_time(mats)
+    _, plain64_rate = one_at_a_time([np.asfortranarray(m.astype(np.float64)) for m in mats])
+    lam = 0.8 * plain_rate
+    gaps = rng.exponential(1.0 / lam, size=len(mats))
+    outs, served_s, st = _serve(mats, gaps, budget_gb, inflight, device_sigma)
+    sat_outs, sat_s, sat_st = _serve(mats, None, budget_gb, inflight, device_sigma)
+    errs = [sigma_err(o[0], ref) for o, ref in zip(outs, refs) if ref is not None]
+    same = all(np.array_equal(a[0], b[0]) for a, b in zip(outs, sat_outs))
     arr = np.cumsum(gaps)
     fin, lat_fifo = 0.0, []
     for a_s, svc_ms in zip(arr, lat_plain):
         fin = max(a_s, fin) + svc_ms / 1e3
         lat_fifo.append((fin - a_s) * 1e3)
-    emit({"suite": "serving", "requests": len(reqs), "arrival_rate_per_s": round(lam, 3),
-          "hbm_budget_gb": args.budget_gb, "max_inflight": args.inflight,
-          "served": {"p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1),
-                     "solves_per_s": round(len(mats) / served_s, 3)},
-          "one_at_a_time": {"p50_ms": round(float(np.percentile(lat_plain, 50)), 1),
-                            "p99_ms": round(float(np.percentile(lat_plain, 99)), 1),
-                            "solves_per_s": round(plain_rate, 3)},
-          "one_at_a_time_fifo_same_arrivals": {"p50_ms": round(float(np.percentile(lat_fifo, 50)), 1),
-                                               "p99_ms": round(float(np.percentile(lat_fifo, 99)), 1),
-                                               "solves_per_s": round(len(mats) / (fin - arr[0]), 3)},
-          "per_request": [{"n": r[0], "dtype": np.dtype(r[1]).name, "served_ms": round(o[1], 1),
-                           "service_alone_ms": round(lp, 1)} for r, o, lp in zip(reqs, outs, lat_plain)],
-          "max_sigma_err_over_sigma_max_geometric_requests": max(errs) if errs else None,
-          "reference": "exact prescribed spectrum (cond 1e4) for the geometric requests"})
+    return {"requests": len(reqs), "n": [r[0] for r in reqs], "gen_s": round(gen_s, 1),
+            "fp32_share": round(sum(r[1] == np.float32 for r in reqs) / len(reqs), 3),
+            "hbm_budget_gb": budget_gb, "max_inflight": inflight, "device_sigma": bool(device_sigma),
+            "poisson": {"arrival_rate_per_s": round(lam, 3), "p50_ms": round(st["p50_ms"], 1),
+                        "p99_ms": round(st["p99_ms"], 1), "solves_per_s": round(len(mats) / served_s, 3)},
+            "saturated": {"p50_ms": round(sat_st["p50_ms"], 1), "p99_ms": round(sat_st["p99_ms"], 1),
+                          "solves_per_s": round(len(mats) / sat_s, 3)},
+            "one_at_a_time": {"p50_ms": round(float(np.percentile(lat_plain, 50)), 1),
+                              "p99_ms": round(float(np.percentile(lat_plain, 99)), 1),
+                              "solves_per_s": round(plain_rate, 3)},
+            "one_at_a_time_fifo_same_arrivals": {"p50_ms": round(float(np.percentile(lat_fifo, 50)), 1),
+                                                 "p99_ms": round(float(np.percentile(lat_fifo, 99)), 1)},
+            "plain_fp64_one_at_a_time_solves_per_s": round(plain64_rate, 3),
+            "saturated_over_plain_fp64": round((len(mats) / sat_s) / plain64_rate, 2),
+            "results_same_poisson_and_saturated": bool(same),
+            "max_sigma_err_over_sigma_max_geometric_requests": max(errs) if errs else None,
+            "reference": "exact prescribed spectrum (cond 1e4) for the geometric requests"}
+
+
+def suite_serving(args):
+    d = serving_run(args.requests, args.nmax, args.budget_gb, args.inflight, args.device_sigma)
+    d["suite"] = "serving"
+    emit(d)
 
 
 def main(argv=None):
@@ -380,6 +416,8 @@ def main(argv=None):
     p.add_argument("--nmax", type=int, default=16384)
     p.add_argument("--budget-gb", type=float, default=64)
     p.add_argument("--inflight", type=int, default=4)
+    p.add_argument("--device-sigma", action="store_true",
+                   help="bidiagonals' singular values by bisection on the device in the serving loop")
     args = ap.parse_args(argv)
     {"plain": suite_plain, "stage2": suite_stage2, "precision": suite_precision, "batched": suite_batched,
      "serving": suite_serving}[args.suite](args)

@@ -124,8 +124,16 @@ struct avs_handle {
     int *dChaseErr = nullptr;
     int *dCnt = nullptr, *dTicket = nullptr, *dStatus = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool have_matrix = false;
-    int last_alg = -1, last_kend = 0;
+    // what the tiles hold: nothing yet, the uploaded matrix, the reduced
+    // matrix after band_k steps of Blockbidiag! (band in place, reflectors
+    // where the reference writes exact zeros), or R and reflectors after
+    // tiled_QR!
+    enum { ST_EMPTY, ST_MATRIX, ST_BAND, ST_QR } state = ST_EMPTY;
+    int band_k = 0;
+    // bisection for the bidiagonal's singular values: device buffers (the
+    // Golub-Kahan tridiagonal's squared off-diagonals, 2n; sigma, n) and their
+    // pinned host staging, allocated on first use and kept with the handle
+    double *dT2 = nullptr, *dSig = nullptr, *hT2 = nullptr, *hSig = nullptr;
     bool timed = false;
     int workers = 0, device = 0;
     long long wallclock_khz = 100000;
@@ -433,11 +441,14 @@ template <typename TS, int B> avs_status run_streams_graph(avs_handle *h)
 namespace avs {
 
 // Singular values of a bidiagonal by bisection on the device (one lane per
-// value), with this thread's stream and buffers (created on first use, grown
-// as needed and kept for the thread's lifetime); LAPACK dbdsdc on the host
-// for n < 128 (the device round trip costs more) or with AVSVD_BDSDC=host.
-avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma)
+// value) on stream s, with the caller's buffers: dT2 (2n doubles), dSig (n)
+// on the device and hT2, hSig (the same sizes, pinned) on the host; LAPACK
+// dbdsdc on the host for n < 128 (the device round trip costs more) or with
+// AVSVD_BDSDC=host.  Allocates nothing on the device path.
+avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma, double *dT2,
+                        double *dSig, double *hT2, double *hSig, void *sv)
 {
+    hipStream_t s = (hipStream_t)sv;
     if (n <= 0 || nkeep <= 0 || nkeep > n) return fail(-1, "bad sizes n=%lld nkeep=%lld", (long long)n,
                                                        (long long)nkeep);
     const char *pol = getenv("AVSVD_BDSDC");
@@ -448,38 +459,25 @@ avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double 
         memcpy(sigma, sg.data(), sizeof(double) * nkeep);
         return AVS_OK;
     }
-    thread_local hipStream_t ts = nullptr;
-    thread_local double *dT2 = nullptr, *dSig = nullptr;
-    thread_local int64_t cap = 0;
-    if (!ts) HIPCHK(hipStreamCreateWithFlags(&ts, hipStreamNonBlocking));
-    if (cap < n) {
-        if (dT2) HIPCHK(hipFree(dT2));
-        if (dSig) HIPCHK(hipFree(dSig));
-        dT2 = dSig = nullptr;
-        cap = 0;
-        HIPCHK(hipMalloc(&dT2, sizeof(double) * 2 * (size_t)n));
-        HIPCHK(hipMalloc(&dSig, sizeof(double) * (size_t)n));
-        cap = n;
-    }
     // squared off-diagonals of the Golub-Kahan tridiagonal, its Gershgorin
     // bound and LAPACK's pivmin
-    std::vector<double> t2(2 * n - 1);
     double bnd = 0.0, tmx = 0.0, prev = 0.0;
     for (int64_t k = 0; k < 2 * n - 1; ++k) {
         const double t = std::fabs((k & 1) ? e[k >> 1] : d[k >> 1]);
-        t2[k] = t * t;
+        hT2[k] = t * t;
         bnd = std::max(bnd, prev + t);
-        tmx = std::max(tmx, t2[k]);
+        tmx = std::max(tmx, hT2[k]);
         prev = t;
     }
     bnd = std::max(bnd, prev);
     const double hi0 = bnd * (1.0 + 1e-14) + 1e-300;
     const double pivmin = 2.2250738585072014e-308 * std::max(1.0, tmx);
-    HIPCHK(hipMemcpyAsync(dT2, t2.data(), sizeof(double) * t2.size(), hipMemcpyHostToDevice, ts));
-    launch_bisect(dT2, n, nkeep, hi0, pivmin, dSig, ts);
+    HIPCHK(hipMemcpyAsync(dT2, hT2, sizeof(double) * (size_t)(2 * n - 1), hipMemcpyHostToDevice, s));
+    launch_bisect(dT2, n, nkeep, hi0, pivmin, dSig, s);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(sigma, dSig, sizeof(double) * nkeep, hipMemcpyDeviceToHost, ts));
-    HIPCHK(hipStreamSynchronize(ts));
+    HIPCHK(hipMemcpyAsync(hSig, dSig, sizeof(double) * nkeep, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(sigma, hSig, sizeof(double) * nkeep);
     return AVS_OK;
 }
 
@@ -582,10 +580,18 @@ avs_status run_alg(avs_handle *h, int alg, int32_t kend)
         return fail(AVS_ERR_UNSUPPORTED,
                     "the persistent dependency-counting executor is disabled: it did not complete on the device "
                     "and its cause is not yet identified; use AVS_SCHED_STREAMS or AVS_SCHED_HOST");
-    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    if (h->state == avs_handle::ST_EMPTY) return fail(AVS_ERR_STATE, "no matrix set on the handle");
     if (kend <= 0 || kend > h->nt) kend = h->nt;
     avs_status st = h->opts.sched == AVS_SCHED_STREAMS ? ensure_phases(h, alg, kend) : ensure_program(h, alg, kend);
     if (st) return st;
+    // after Blockbidiag! the reference's matrix holds the band with explicit
+    // zeros (tiledalgos.jl:46,58); the tiles hold reflectors there, so write
+    // the zeros first and run on the same matrix the reference would
+    if (h->state == avs_handle::ST_BAND) {
+        if (h->dtype == AVS_F64) launch_zero_band<double>(h->dTiles, h->nt, h->b, h->band_k, h->stream);
+        else launch_zero_band<float>(h->dTiles, h->nt, h->b, h->band_k, h->stream);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     st = dispatch_tb(h->dtype, h->b, [&](auto tp, auto bc) -> avs_status {
         typedef std::remove_pointer_t<decltype(tp)> TS;
@@ -597,8 +603,8 @@ avs_status run_alg(avs_handle *h, int alg, int32_t kend)
     if (st) return st;
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
-    h->last_alg = alg;
-    h->last_kend = kend;
+    h->state = alg == ALG_BLOCKBIDIAG ? avs_handle::ST_BAND : avs_handle::ST_QR;
+    h->band_k = kend;
     return AVS_OK;
 }
 
@@ -702,6 +708,10 @@ avs_status avs_destroy(avs_handle *h)
     hipFree(h->dCnt);
     hipFree(h->dTicket);
     hipFree(h->dStatus);
+    hipFree(h->dT2);
+    hipFree(h->dSig);
+    if (h->hT2) hipHostFree(h->hT2);
+    if (h->hSig) hipHostFree(h->hSig);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     for (hipEvent_t e : h->pev) hipEventDestroy(e);
@@ -729,8 +739,8 @@ avs_status avs_set_matrix_device(avs_handle *h, const void *dA, int64_t lda)
     if (h->dtype == AVS_F64) launch_pack<double>(dA, lda, h->dTiles, h->nt, h->b, h->stream);
     else launch_pack<float>(dA, lda, h->dTiles, h->nt, h->b, h->stream);
     HIPCHK(hipGetLastError());
-    h->have_matrix = true;
-    h->last_alg = -1;
+    h->state = avs_handle::ST_MATRIX;
+    h->band_k = 0;
     return AVS_OK;
 }
 
@@ -765,9 +775,9 @@ avs_status avs_get_matrix_device(avs_handle *h, void *dA, int64_t lda)
     if (!h) return fail(-1, "null handle");
     if (!dA) return fail(-2, "null matrix");
     if (lda < h->n) return fail(-3, "lda < n");
-    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
-    const int mode = h->last_alg == ALG_BLOCKBIDIAG ? 1 : 0;
-    const int kend = h->last_alg == ALG_BLOCKBIDIAG ? h->last_kend : 0;
+    if (h->state == avs_handle::ST_EMPTY) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    const int mode = h->state == avs_handle::ST_BAND ? 1 : 0;
+    const int kend = h->state == avs_handle::ST_BAND ? h->band_k : 0;
     if (h->dtype == AVS_F64) launch_unpack<double>(h->dTiles, dA, lda, h->nt, h->b, mode, kend, h->stream);
     else launch_unpack<float>(h->dTiles, dA, lda, h->nt, h->b, mode, kend, h->stream);
     HIPCHK(hipGetLastError());
@@ -875,7 +885,7 @@ avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband)
     if (!h) return fail(-1, "null handle");
     if (!band) return fail(-2, "null band");
     if (ldband < h->b + 1) return fail(-3, "ldband < b+1");
-    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    if (h->state == avs_handle::ST_EMPTY) return fail(AVS_ERR_STATE, "no matrix set on the handle");
     if (!h->dBand) HIPCHK(hipMalloc(&h->dBand, sizeof(double) * (size_t)(h->b + 1) * h->n));
     if (h->dtype == AVS_F64) launch_band<double>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
     else launch_band<float>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
@@ -929,9 +939,15 @@ avs_status avs_svdvals(avs_handle *h, double *sigma)
 {
     if (!h) return fail(-1, "null handle");
     if (!sigma) return fail(-2, "null sigma");
-    if (!h->have_matrix) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    if (h->state == avs_handle::ST_EMPTY) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    // tiled_QR! has replaced the matrix by R and its reflectors, whose
+    // singular values are not the matrix's: the caller must upload it again
+    if (h->state == avs_handle::ST_QR)
+        return fail(AVS_ERR_STATE, "the tiles hold tiled_QR!'s R and reflectors, not the matrix: set the matrix again");
     avs_status st;
-    if (h->last_alg != ALG_BLOCKBIDIAG || h->last_kend != h->nt) {
+    // the uploaded matrix, or a partial Blockbidiag! (its band and trailing
+    // block are an orthogonal transform of the matrix: same singular values)
+    if (h->state != avs_handle::ST_BAND || h->band_k != h->nt) {
         st = avs_blockbidiag(h, 0);
         if (st) return st;
     }
@@ -947,7 +963,13 @@ avs_status avs_svdvals(avs_handle *h, double *sigma)
         st = bidiagonalize_band_auto(n, b, band.data(), b + 1, d.data(), e.data());
     }
     if (st) return st;
-    return bidiag_sigma(n, n, d.data(), e.data(), sigma);
+    if (!h->dT2) {
+        HIPCHK(hipMalloc(&h->dT2, sizeof(double) * 2 * (size_t)n));
+        HIPCHK(hipMalloc(&h->dSig, sizeof(double) * (size_t)n));
+        HIPCHK(hipHostMalloc((void **)&h->hT2, sizeof(double) * 2 * (size_t)n, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&h->hSig, sizeof(double) * (size_t)n, hipHostMallocDefault));
+    }
+    return bidiag_sigma(n, n, d.data(), e.data(), sigma, h->dT2, h->dSig, h->hT2, h->hSig, h->stream);
 }
 
 // ------------------------------------------------------------ tile operators

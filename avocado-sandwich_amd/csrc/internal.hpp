@@ -15,9 +15,12 @@ avs_status bidiagonalize_band_one(int64_t n, int32_t bw, const double *band, int
                                   int nthreads);
 avs_status bidiagonalize_band_auto(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e);
 // singular values of the upper bidiagonal (d, e) of order n, the nkeep
-// largest, descending: bisection on the device (bidiag_sigma in api.cpp), or
-// LAPACK dbdsdc on the host for n < 128 or with AVSVD_BDSDC=host
-avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma);
+// largest, descending: bisection on the device on stream `s` with the
+// caller's buffers (dT2: 2n doubles, dSig: n on the device; hT2, hSig the same
+// sizes in pinned host memory), or LAPACK dbdsdc on the host for n < 128 or
+// with AVSVD_BDSDC=host (bidiag_sigma in api.cpp)
+avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma, double *dT2,
+                        double *dSig, double *hT2, double *hSig, void *s);
 // band -> bidiagonal by Householder bulge chasing, sweeps pipelined over
 // nthreads host threads (bandbrd.cpp); the result does not depend on nthreads
 avs_status bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,

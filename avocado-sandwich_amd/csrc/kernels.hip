@@ -44,6 +44,23 @@ __global__ void __launch_bounds__(256) unpack_kernel(const TS *__restrict__ tile
     }
 }
 
+// In place, the zeros the reference's Blockbidiag! writes (tiledalgos.jl:46,58)
+// for its first kend steps: strict lower part of the diagonal tiles, strict
+// upper part of the super-diagonal tiles, every other tile of those rows and
+// columns; the trailing block is left as it is
+template <typename TS>
+__global__ void __launch_bounds__(256) zero_band_kernel(TS *__restrict__ tiles, int nt, int b, int kend)
+{
+    const int i = blockIdx.x, j = blockIdx.y;
+    if (i >= kend && j >= kend) return;
+    TS *t = tiles + ((long)j * nt + i) * b * b;
+    for (int idx = threadIdx.x; idx < b * b; idx += blockDim.x) {
+        const int r = idx % b, c = idx / b;
+        const bool keep = (j == i) ? r <= c : (j == i + 1 ? r >= c : false);
+        if (!keep) t[idx] = (TS)0;
+    }
+}
+
 // band[(b + i - j) + j*ldband] = A(i,j), max(0,j-b) <= i <= j, as FP64
 template <typename TS>
 __global__ void __launch_bounds__(256) band_kernel(const TS *__restrict__ tiles, double *__restrict__ band,
@@ -485,6 +502,10 @@ void launch_unpack(const void *tiles, void *A, long lda, int nt, int b, int mode
     hipLaunchKernelGGL(unpack_kernel<TS>, dim3(nt, nt), dim3(256), 0, s, (const TS *)tiles, (TS *)A, lda, nt, b,
                        mode, kend);
 }
+template <typename TS> void launch_zero_band(void *tiles, int nt, int b, int kend, hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_band_kernel<TS>, dim3(nt, nt), dim3(256), 0, s, (TS *)tiles, nt, b, kend);
+}
 template <typename TS> void launch_band(const void *tiles, double *band, long ldband, int nt, int b, hipStream_t s)
 {
     hipLaunchKernelGGL(band_kernel<TS>, dim3((unsigned)((long)nt * b)), dim3(256), 0, s, (const TS *)tiles, band,
@@ -608,6 +629,8 @@ template void launch_pack<float>(const void *, long, void *, int, int, hipStream
 template void launch_unpack<double>(const void *, void *, long, int, int, int, int, hipStream_t);
 template void launch_unpack<float>(const void *, void *, long, int, int, int, int, hipStream_t);
 template void launch_band<double>(const void *, double *, long, int, int, hipStream_t);
+template void launch_zero_band<double>(void *, int, int, int, hipStream_t);
+template void launch_zero_band<float>(void *, int, int, int, hipStream_t);
 template void launch_batch_pack<double>(const void *, const long *, void *, const long *, const int *, const long *,
                                         long, int, int, hipStream_t);
 template void launch_batch_pack<float>(const void *, const long *, void *, const long *, const int *, const long *,

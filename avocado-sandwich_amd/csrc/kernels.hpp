@@ -12,6 +12,7 @@ template <typename TS> void launch_pack(const void *A, long lda, void *tiles, in
 template <typename TS>
 void launch_unpack(const void *tiles, void *A, long lda, int nt, int b, int mode, int kend, hipStream_t s);
 template <typename TS> void launch_band(const void *tiles, double *band, long ldband, int nt, int b, hipStream_t s);
+template <typename TS> void launch_zero_band(void *tiles, int nt, int b, int kend, hipStream_t s);
 
 template <typename TS, int B>
 void launch_panel(int mode, void *top, long ldt, void *bot, long ldb, int tr, double *tau, double *tjj,

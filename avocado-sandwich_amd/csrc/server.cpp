@@ -1,12 +1,20 @@
 // server.cpp -- continuous-batching svdvals request loop.
 //
-// Requests (any n, FP32 or FP64) are accepted as they arrive.  A request is
-// admitted to the device when the HBM budget has room for its working set
-// (tiles, staging copy, factor slots); admission is first-come first-served.
-// Each of `max_inflight` device workers owns its own HIP stream, so the band
-// reductions of admitted requests overlap on the GPU while later requests
-// wait for memory.  Only the (b+1) x N band comes back over PCIe; the host
-// stage (block bulge chasing + dbdsdc, FP64) runs on a pool of host workers.
+// Requests (any n, FP32 or FP64) are accepted as they arrive and admitted to
+// the device first-come first-served.  The device state lives in a pool of
+// handles (tile-major working copy, upload staging, factor slots, band
+// buffers), one per (dtype, padded order N, tile size b); a handle is created
+// when a request of its shape is admitted and none is idle, kept after the
+// request and reused by the next request of the same shape, so a steady
+// stream of requests allocates nothing.  Every pooled handle counts against
+// the HBM budget, idle or busy; when a new shape does not fit, idle handles
+// of other shapes are destroyed (least recently used first).  Each of
+// `max_inflight` device workers runs one request at a time on its handle's
+// streams, so the band reductions of admitted requests overlap on the GPU.
+// Only the (b+1) x N band comes back over PCIe; the host stage (bulge chasing
+// in FP64, then the bidiagonal's singular values by bisection on the device
+// or LAPACK dbdsdc) runs on a pool of host workers, each with its own stream
+// and bisection buffers sized once for the largest order the budget admits.
 // Latency is arrival -> singular values ready.
 #include <hip/hip_runtime_api.h>
 
@@ -92,8 +100,30 @@ size_t footprint_of(int dtype, int64_t N, int b)
 
 } // namespace
 
+// one pooled handle
+struct Slot {
+    avs_handle *h = nullptr;
+    int dtype = AVS_F64;
+    int64_t N = 0;
+    int b = 0;
+    size_t bytes = 0;
+    bool busy = false;
+    uint64_t last_use = 0;
+};
+
+// per host worker: stream and bisection buffers (2*nmax and nmax doubles on
+// the device and pinned on the host), allocated at server creation
+struct HostBufs {
+    hipStream_t s = nullptr;
+    double *dT2 = nullptr, *dSig = nullptr, *hT2 = nullptr, *hSig = nullptr;
+};
+
 struct avs_server {
     avs_server_opts opts{};
+    std::vector<Slot> pool;
+    uint64_t use_clock = 0;
+    int64_t nmax = 0;
+    std::vector<HostBufs> hbufs;
     std::mutex mu;
     std::condition_variable cv_dev, cv_host, cv_done;
     std::deque<std::shared_ptr<Request>> pending, host_q;
@@ -109,29 +139,99 @@ struct avs_server {
 
 namespace {
 
+void free_host_bufs(avs_server *S)
+{
+    for (HostBufs &hb : S->hbufs) {
+        if (hb.s) hipStreamSynchronize(hb.s);
+        hipFree(hb.dT2);
+        hipFree(hb.dSig);
+        if (hb.hT2) hipHostFree(hb.hT2);
+        if (hb.hSig) hipHostFree(hb.hSig);
+        if (hb.s) hipStreamDestroy(hb.s);
+        hb = HostBufs();
+    }
+}
+
+// Takes the request at the head of the queue once a pooled handle of its
+// shape is idle or the budget has room for a new one (after evicting idle
+// handles of other shapes, least recently used first).  Called with S->mu
+// held; returns the slot index or -1 (wait).  Handles to destroy are moved to
+// `evict` (destroyed by the caller without the lock).
+int admit(avs_server *S, const Request &r, std::vector<avs_handle *> &evict)
+{
+    int idle = -1;
+    for (size_t i = 0; i < S->pool.size(); ++i) {
+        const Slot &x = S->pool[i];
+        if (!x.busy && x.dtype == r.dtype && x.N == r.N && x.b == r.b) {
+            idle = (int)i;
+            break;
+        }
+    }
+    if (idle >= 0) {
+        S->pool[idle].busy = true;
+        S->pool[idle].last_use = ++S->use_clock;
+        return idle;
+    }
+    const size_t budget = (size_t)S->opts.hbm_budget_bytes;
+    size_t idle_bytes = 0;
+    for (const Slot &x : S->pool)
+        if (!x.busy) idle_bytes += x.bytes;
+    if (S->used - idle_bytes + r.footprint > budget) return -1; // wait for busy handles to finish
+    while (S->used + r.footprint > budget) {
+        int lru = -1;
+        for (size_t i = 0; i < S->pool.size(); ++i)
+            if (!S->pool[i].busy && (lru < 0 || S->pool[i].last_use < S->pool[lru].last_use)) lru = (int)i;
+        evict.push_back(S->pool[lru].h);
+        S->used -= S->pool[lru].bytes;
+        S->pool.erase(S->pool.begin() + lru);
+    }
+    Slot x;
+    x.dtype = r.dtype;
+    x.N = r.N;
+    x.b = r.b;
+    x.bytes = r.footprint;
+    x.busy = true;
+    x.last_use = ++S->use_clock;
+    S->used += x.bytes;
+    S->pool.push_back(x);
+    return (int)S->pool.size() - 1;
+}
+
 void device_worker(avs_server *S)
 {
     for (;;) {
         std::shared_ptr<Request> r;
+        std::vector<avs_handle *> evict;
+        avs_handle *h = nullptr;
+        size_t key = 0; // slot identity: its position can move when others are evicted
         {
             std::unique_lock<std::mutex> lk(S->mu);
+            int slot = -1;
             S->cv_dev.wait(lk, [&] {
-                return S->stop || (!S->pending.empty() &&
-                                   S->used + S->pending.front()->footprint <= (size_t)S->opts.hbm_budget_bytes);
+                if (S->stop) return true;
+                if (S->pending.empty()) return false;
+                slot = admit(S, *S->pending.front(), evict);
+                return slot >= 0;
             });
-            if (S->stop) return;
+            if (S->stop) {
+                lk.unlock();
+                for (avs_handle *e : evict) avs_destroy(e);
+                return;
+            }
             r = S->pending.front();
             S->pending.pop_front();
-            S->used += r->footprint;
             r->t_admit = now_ms();
+            h = S->pool[slot].h;
+            key = S->pool[slot].last_use;
         }
-        avs_opts o;
-        avs_default_opts(&o);
-        o.update_mode = S->opts.update_mode;
-        const long nt = (long)(r->N / r->b);
-        o.workers = (int32_t)std::max(16L, std::min(256L, nt * nt / 2));
-        avs_handle *h = nullptr;
-        avs_status st = avs_create(&h, r->dtype, r->N, r->b, &o);
+        for (avs_handle *e : evict) avs_destroy(e);
+        avs_status st = AVS_OK;
+        if (!h) {
+            avs_opts o;
+            avs_default_opts(&o);
+            o.update_mode = S->opts.update_mode;
+            st = avs_create(&h, r->dtype, r->N, r->b, &o);
+        }
         if (!st) st = avs_set_matrix_host_sub(h, r->A.data(), r->n, r->n);
         if (!st) st = avs_blockbidiag(h, 0);
         if (!st) {
@@ -139,23 +239,38 @@ void device_worker(avs_server *S)
             st = avs_get_band(h, r->band.data(), r->b + 1);
         }
         std::string msg = st ? avs_last_error() : "";
-        if (h) avs_destroy(h);
         r->t_band = now_ms();
         std::vector<char>().swap(r->A);
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            S->used -= r->footprint;
+            for (size_t i = 0; i < S->pool.size(); ++i) {
+                Slot &x = S->pool[i];
+                if (x.busy && x.last_use == key) {
+                    if (st) { // a failed handle is not reused
+                        if (h) evict.assign(1, h);
+                        S->used -= x.bytes;
+                        S->pool.erase(S->pool.begin() + i);
+                    } else {
+                        x.h = h;
+                        x.busy = false;
+                    }
+                    break;
+                }
+            }
             r->status = st;
             r->err = msg;
             S->host_q.push_back(r);
         }
+        if (st)
+            for (avs_handle *e : evict) avs_destroy(e);
         S->cv_dev.notify_all();
         S->cv_host.notify_one();
     }
 }
 
-void host_worker(avs_server *S)
+void host_worker(avs_server *S, int wid)
 {
+    const HostBufs &hb = S->hbufs[wid];
     for (;;) {
         std::shared_ptr<Request> r;
         {
@@ -172,11 +287,16 @@ void host_worker(avs_server *S)
             // requests set the p99; the sweeps pipeline well there)
             avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, d.data(), e.data(),
                                                         N >= 4096 ? 4 : 1);
-            // LAPACK dbdsdc on this host thread: with the device bisection
-            // here (avs::bidiag_sigma, one stream per host worker) a 48-request
-            // run with 16 host workers stopped making progress once, cause
-            // not identified, so the serving loop does not use it
-            if (!st) st = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
+            // the bidiagonal's singular values: bisection on the device with
+            // this worker's stream and buffers (nothing allocated here), or
+            // LAPACK dbdsdc on this thread
+            if (!st) {
+                if (S->opts.device_sigma && N <= S->nmax)
+                    st = avs::bidiag_sigma(N, N, d.data(), e.data(), sg.data(), hb.dT2, hb.dSig, hb.hT2, hb.hSig,
+                                           hb.s);
+                else
+                    st = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
+            }
             if (st) {
                 r->status = st;
                 r->err = avs_last_error();
@@ -253,8 +373,32 @@ avs_status avs_server_create(avs_server **sp, const avs_server_opts *opts)
     if (S->opts.hbm_budget_bytes <= 0) return fail(-2, "hbm_budget_bytes must be positive");
     if (S->opts.max_inflight <= 0) S->opts.max_inflight = 4;
     if (S->opts.host_threads <= 0) S->opts.host_threads = avs::host_thread_count();
+    for (int32_t r : S->opts.reserved)
+        if (r != 0) return fail(-2, "opts.reserved must be zero");
+    // the largest padded order the budget admits (FP32, the smallest
+    // footprint per element), which sizes the bisection buffers
+    int64_t nmax = 128;
+    while (footprint_of(AVS_F32, 2 * nmax, 128) <= (size_t)S->opts.hbm_budget_bytes) nmax *= 2;
+    while (footprint_of(AVS_F32, nmax + 128, 128) <= (size_t)S->opts.hbm_budget_bytes) nmax += 128;
+    S->nmax = nmax;
+    if (S->opts.device_sigma) {
+        S->hbufs.resize(S->opts.host_threads);
+        for (HostBufs &hb : S->hbufs) {
+            hipError_t e = hipStreamCreateWithFlags(&hb.s, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipMalloc(&hb.dT2, sizeof(double) * 2 * (size_t)nmax);
+            if (e == hipSuccess) e = hipMalloc(&hb.dSig, sizeof(double) * (size_t)nmax);
+            if (e == hipSuccess) e = hipHostMalloc((void **)&hb.hT2, sizeof(double) * 2 * (size_t)nmax, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc((void **)&hb.hSig, sizeof(double) * (size_t)nmax, hipHostMallocDefault);
+            if (e != hipSuccess) {
+                free_host_bufs(S.get());
+                return fail(AVS_ERR_HIP, "server bisection buffers: %s", hipGetErrorString(e));
+            }
+        }
+    } else {
+        S->hbufs.resize(S->opts.host_threads);
+    }
     for (int i = 0; i < S->opts.max_inflight; ++i) S->dev_workers.emplace_back(device_worker, S.get());
-    for (int i = 0; i < S->opts.host_threads; ++i) S->host_workers.emplace_back(host_worker, S.get());
+    for (int i = 0; i < S->opts.host_threads; ++i) S->host_workers.emplace_back(host_worker, S.get(), i);
     *sp = S.release();
     return AVS_OK;
 }
@@ -352,6 +496,9 @@ avs_status avs_server_destroy(avs_server *S)
     S->cv_host.notify_all();
     for (auto &t : S->dev_workers) t.join();
     for (auto &t : S->host_workers) t.join();
+    for (Slot &x : S->pool)
+        if (x.h) avs_destroy(x.h);
+    free_host_bufs(S);
     delete S;
     return AVS_OK;
 }

@@ -1,12 +1,19 @@
 #!/usr/bin/env python3
-"""bench.py -- n=16384 FP64 band reduction (Blockbidiag!) on one MI355X.
+"""bench.py -- n=16384 FP64 band reduction (Blockbidiag!) on one MI355X, plus
+the batched and served svdvals forms and the reference's CPU path.
 
 One step = one two-sided tiled QR/LQ reduction of a dense n x n matrix to
 block-bidiagonal form (the reference's Blockbidiag!, KAbasedSVD/src/
 tiledalgos.jl:17-34): repack the pristine column-major input (resident in HBM)
 into the tile-major working copy, then run the tile DAG.  Algorithmic work per
 step = (8/3) n^3 flops (the gebrd count; T factors, tree merges and padding
-are not counted).  Prints one JSON line.
+are not counted).  After the timed steps: the timed handle's singular values
+(default svdvals path) against the cached FP64 dgesdd vector; the batched form
+(4096 FP32 matrices of mixed order 64..1024: svdvals/s, sigma error); the
+served form (Poisson and saturated request streams of mixed n and precision:
+p50/p99 latency, solves/s, against the one-at-a-time plain path); the CPU
+baseline (the reference's CPU plain path restated in C, stage by stage).
+Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384] [--b 128]
 """
@@ -29,18 +36,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, b):
-    """The oracle's C restatement of the reference's Blockbidiag! (one core),
-    timed on a bounded sample: one full reduction at size n."""
+def cpu_baseline(ns=(1024, 4096), b=64):
+    """The reference's CPU plain path (CUDAbasedSVD/SVD_GPU.jl:67-214 band
+    reduction with LAPACK QR per cached block, then the block bulge chasing of
+    KAbasedSVD/src/bulgechasing.jl:1-42 and dbdsdc, :50-53), restated in C in
+    oracle/ and timed stage by stage on this host, with host FP64 dgesdd beside
+    it.  The block loop runs on `threads` host threads, one BLAS thread each.
+    value = (8/3) n^3 / band-stage seconds at the largest n."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import oracle
-    a = oracle.gaussian_matrix(n, SEED)
-    t0 = time.perf_counter()
-    oracle.blockbidiag(a, b)
-    dt = time.perf_counter() - t0
-    return {"value": (8.0 / 3.0) * n ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-            "sample": "oracle Blockbidiag! (reference loop order, FP64, 1 thread) on one n=%d b=%d Gaussian "
-                      "matrix: %.2f s" % (n, b, dt), "seconds": dt}
+    threads = max(1, min(16, os.cpu_count() or 1))
+    stages = {}
+    for n in ns:
+        a = oracle.gaussian_matrix(n, SEED + (1 if n == 1024 else 0))
+        t0 = time.perf_counter()
+        band = oracle.band_bidiagonal_cpu(a, b, threads)
+        t1 = time.perf_counter()
+        bd = oracle.bidiagonalize(band, b)
+        t2 = time.perf_counter()
+        sig = oracle.bdsdc(np.diag(bd).copy(), np.diag(bd, 1).copy())
+        t3 = time.perf_counter()
+        ref = oracle.svdvals_ref(a)
+        t4 = time.perf_counter()
+        stages[str(n)] = {"band_s": round(t1 - t0, 3), "band_tflops": round((8.0 / 3.0) * n ** 3 / (t1 - t0) / 1e12, 5),
+                          "bulge_chasing_s": round(t2 - t1, 3), "bdsdc_s": round(t3 - t2, 4),
+                          "gesdd_s": round(t4 - t3, 3), "sigma_err_vs_gesdd": oracle.sigma_err(sig, ref)}
+        log("cpu baseline n=%d: %s" % (n, stages[str(n)]))
+    top = stages[str(max(ns))]
+    return {"value": top["band_tflops"], "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "sample": "reference CPU plain path (BandBidiagonal!, CUDAbasedSVD/SVD_GPU.jl:67-214; LAPACK dgeqrt/"
+                      "dgemqrt per cached block, block loop on %d threads with one BLAS thread each) restated in C, "
+                      "FP64, b=%d, Gaussian n=%s; value = band stage at n=%d; bulge chasing "
+                      "(bulgechasing.jl:1-42), dbdsdc and host dgesdd timed beside it" %
+                      (threads, b, "/".join(str(x) for x in ns), max(ns), max(ns)),
+            "stages": stages, "host_cpus": os.cpu_count()}
 
 
 def panel_rate(n, b, tree, domain, dtype, panel_ms):
@@ -78,10 +108,14 @@ def main():
                     help="CUs (the same number per XCD) the CU-masked update stream leaves to the panels")
     ap.add_argument("--masked-modes", type=int, default=2,
                     help="update modes on the CU-masked stream: bit 0 GEQRT's, 1 TSQRT's, 2 merges' (0 = all)")
-    ap.add_argument("--cpu-n", type=int, default=1536)
+    ap.add_argument("--cpu-ns", type=int, nargs="+", default=[1024, 4096])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--check-n", type=int, default=4096,
-                    help="size of the sigma check against the cached FP64 dgesdd vector (0 = skip)")
+    ap.add_argument("--check-n", type=int, default=16384,
+                    help="sigma check against the cached FP64 dgesdd vector: the timed handle's own result when "
+                         "equal to --n, else a separate run at this size (0 = skip)")
+    ap.add_argument("--batched-count", type=int, default=4096, help="batched form: matrices (0 = skip)")
+    ap.add_argument("--serving-requests", type=int, default=24, help="served form: requests (0 = skip)")
+    ap.add_argument("--serving-nmax", type=int, default=16384)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (a known-traffic kernel for scaling "
                          "hardware byte counters)")
@@ -188,21 +222,34 @@ def main():
     band = np.zeros((b + 1, n), order="F")
     check(L.avs_get_band(h, dptr(band), b + 1))
     fro_err = abs(float(np.linalg.norm(band)) - fro) / fro
-    L.avs_destroy(h)
+    del band
 
     sigma_check = None
     ref_path = os.path.join(ROOT, "tests", "golden", "sigma_gauss_n%d_s%d.npy" % (args.check_n, SEED))
     if args.check_n and os.path.exists(ref_path):
-        a_chk = avsvd.gaussian_matrix(args.check_n, SEED, dtype=dt)
-        T = avsvd.TiledMatrix(a_chk, b, b, sched=o.sched, tree=o.tree, domain=args.domain)
-        t_s = time.perf_counter()
-        sig = T.svdvals()
-        t_s = time.perf_counter() - t_s
         ref = np.load(ref_path)
+        t_s = time.perf_counter()
+        if args.check_n == n:
+            # the timed handle's own band (same options as the timed steps), then the default svdvals path
+            sig = np.empty(n)
+            check(L.avs_svdvals(h, dptr(sig)))
+            what = "the timed handle's band, then the default svdvals path"
+        else:
+            a_chk = avsvd.gaussian_matrix(args.check_n, SEED, dtype=dt)
+            T = avsvd.TiledMatrix(a_chk, b, b, sched=o.sched, tree=o.tree, domain=args.domain,
+                                  reserve_cus=o.reserve_cus, masked_modes=o.masked_modes)
+            sig = T.svdvals()
+            del T
+            what = "a separate handle with the same options"
+        t_s = time.perf_counter() - t_s
         err = float(np.max(np.abs(np.sort(sig)[::-1] - ref)) / ref[0])
-        sigma_check = {"n": args.check_n, "max_abs_err_over_sigma_max": err,
-                       "vs": "FP64 LAPACK dgesdd of the same seeded matrix", "svdvals_seconds": t_s}
-        log("sigma check n=%d: %.3e" % (args.check_n, err))
+        sigma_check = {"n": args.check_n, "max_abs_err_over_sigma_max": err, "bar": 1e-12 if args.dtype == "f64" else 1e-5,
+                       "vs": "FP64 LAPACK dgesdd of the same seeded matrix (cached)", "what": what,
+                       "svdvals_seconds": round(t_s, 3)}
+        log("sigma check n=%d: %.3e (%.2f s)" % (args.check_n, err, t_s))
+    L.avs_destroy(h)
+    del dA
+    torch.cuda.empty_cache()
 
     if args.copy_calibration:
         nb = args.copy_calibration << 20
@@ -220,10 +267,26 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    batched = serving = None
+    from avsvd import suites
+    if args.batched_count:
+        log("batched form: %d FP32 matrices" % args.batched_count)
+        try:
+            batched = suites.batched_run(args.batched_count, 0, plain_sample=32, lapack_sample=256)
+        except Exception as e:  # the headline line is still printed
+            batched = {"error": repr(e)}
+        torch.cuda.empty_cache()
+    if args.serving_requests:
+        log("served form: %d requests" % args.serving_requests)
+        try:
+            serving = suites.serving_run(args.serving_requests, args.serving_nmax, budget_gb=64, inflight=4)
+        except Exception as e:
+            serving = {"error": repr(e)}
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline: oracle n=%d" % args.cpu_n)
-        cpu = cpu_baseline(args.cpu_n, b)
+        cpu = cpu_baseline(tuple(args.cpu_ns), 64)
 
     out = {
         "metric": METRIC,
@@ -256,6 +319,8 @@ def main():
         "panels": panel_rate(n, b, o.tree, args.domain, args.dtype, upd["panel_ms"] / args.steps),
         "band_fro_rel_err": fro_err,
         "sigma_check": sigma_check,
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

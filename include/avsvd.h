@@ -282,7 +282,11 @@ typedef struct avs_server_opts {
     int32_t host_threads;     /* host-stage workers */
     int32_t b_f32, b_f64;     /* tile sizes (0 = the tile-size table below) */
     int32_t update_mode;      /* avs_update_mode for FP32 requests */
-    int32_t reserved[7];
+    int32_t device_sigma;     /* != 0: the bidiagonal's singular values by bisection on the
+                                 device (each host worker has its own stream and buffers,
+                                 allocated at server creation); 0: LAPACK dbdsdc on the
+                                 host worker */
+    int32_t reserved[6];      /* must be 0 */
 } avs_server_opts;
 void avs_server_default_opts(avs_server_opts *opts);
 avs_status avs_server_create(avs_server **server, const avs_server_opts *opts);

@@ -43,6 +43,12 @@ def lib():
         L.or_bidiagonalize.argtypes = [l, l, _dp]
         L.or_block_bidiagonalize.argtypes = [_dp, l, l, l]
         L.or_philox_normal.argtypes = [ctypes.c_uint64, l, l, d, _dp]
+        L.cp_init.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.cp_init.restype = ctypes.c_int
+        L.cp_blas_threads.argtypes = [i]
+        L.cp_blas_threads.restype = ctypes.c_int
+        L.cp_band_bidiagonal.argtypes = [l, i, _dp, i]
+        L.cp_band_bidiagonal.restype = ctypes.c_int
         for f in ("or_geqrt", "or_tsqrt", "or_unmqr", "or_tsmqr", "or_blockbidiag",
                   "or_tiled_qr", "or_bidiagonalize", "or_block_bidiagonalize",
                   "or_philox_normal"):
@@ -183,6 +189,39 @@ def bdsdc(d, e):
     if info.value != 0:
         raise RuntimeError("dbdsdc info=%d" % info.value)
     return dd
+
+
+_CP_READY = False
+
+
+def _cp_ready():
+    global _CP_READY
+    if not _CP_READY:
+        lapack()
+        st = lib().cp_init(_LAPACK._name.encode(), b"scipy_")
+        if st:
+            raise RuntimeError("cpu_plain: LAPACK dgeqrt/dgemqrt not found (%d)" % st)
+        _CP_READY = True
+
+
+def band_bidiagonal_cpu(a, b, nthreads=1):
+    """BandBidiagonal!(A, b, b, nt, nt, false), CUDAbasedSVD/SVD_GPU.jl:67-214:
+    the reference's CPU plain path (LAPACK QR per cached 2b x b block, Q
+    applied to the other blocks; the block loop over `nthreads` threads with a
+    single-threaded BLAS).  Returns the reduced n x n matrix."""
+    _cp_ready()
+    a = _fortran(a)
+    n = a.shape[0]
+    assert a.shape == (n, n) and n % b == 0
+    prev = lib().cp_blas_threads(1)
+    try:
+        st = lib().cp_band_bidiagonal(n, b, _p(a), nthreads)
+    finally:
+        if prev > 0:
+            lib().cp_blas_threads(prev)
+    if st:
+        raise RuntimeError("cp_band_bidiagonal status %d" % st)
+    return a
 
 
 def svdvals_host_stage(band_matrix, bw):

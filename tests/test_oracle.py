@@ -163,3 +163,26 @@ def test_philox_known_answer():
     r = np.sqrt(-2 * np.log(u1))
     exp = [r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)]
     np.testing.assert_allclose(O.philox_normal(0, 2), exp, rtol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# the reference's CPU plain path (CUDAbasedSVD/SVD_GPU.jl:67-214)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("n,b", [(128, 32), (256, 32), (512, 64)])
+def test_cpu_plain_path_matches_tile_oracle(n, b):
+    """BandBidiagonal! with LAPACK QR of each stacked 2b x b block gives the
+    same band as the tile kernels' flat-tree Blockbidiag! (QR of [R; A] by
+    dgeqrt and by TSQRT are the same Householder factorization), a band with
+    exact zeros outside it, and sigma(A)."""
+    a = O.gaussian_matrix(n, 20261203 + 5 * n + b)
+    r = O.band_bidiagonal_cpu(a, b, 1)
+    ref = O.blockbidiag(a, b)
+    assert np.abs(r - ref).max() <= 1e-12 * np.sqrt(n) * np.abs(ref).max()
+    assert np.array_equal(np.tril(r, -1), np.zeros_like(r))
+    assert np.array_equal(np.triu(r, b + 1), np.zeros_like(r))
+    assert O.sigma_err(O.svdvals_ref(r), O.svdvals_ref(a)) <= 10 * n * np.finfo(np.float64).eps
+
+
+def test_cpu_plain_path_thread_count_independent():
+    a = O.gaussian_matrix(384, 20261203 + 9)
+    np.testing.assert_array_equal(O.band_bidiagonal_cpu(a, 32, 1), O.band_bidiagonal_cpu(a, 32, 5))

@@ -106,7 +106,10 @@ void avs_default_opts(avs_opts *opts);
 avs_status avs_create(avs_handle **h, int32_t dtype, int64_t n, int32_t b, const avs_opts *opts);
 avs_status avs_destroy(avs_handle *h);
 
-/* Load the matrix (element type = the handle's dtype), column-major, lda >= n. */
+/* Load the matrix (element type = the handle's dtype), column-major, lda >= n.
+ * The copy runs on the handle's stream, which does not wait for other
+ * streams: a device matrix must be complete when the call is made (or be
+ * produced on the stream given in avs_opts.stream). */
 avs_status avs_set_matrix_host(avs_handle *h, const void *A, int64_t lda);
 avs_status avs_set_matrix_device(avs_handle *h, const void *dA, int64_t lda);
 /* Load an m x m matrix (m <= n) into the leading block and zero the rest:

@@ -108,6 +108,7 @@ def test_full_size_gaussian_vs_cached_gesdd(gpu, golden_dir, n, b, dt):
     a = avsvd.gaussian_matrix(n, BASE, dtype=dt)
     fro = float(np.linalg.norm(a.astype(np.float64)))
     t = torch.from_numpy(a.T).to(gpu)
+    torch.cuda.synchronize()
     del a
     R = Reducer(n, b, "f64" if dt == np.float64 else "f32")
     R.reduce(t.data_ptr(), reps=1)
@@ -133,7 +134,10 @@ def test_geometric_spectrum_n4096_vs_exact(gpu, b, dt):
     V and s from 1 down to 1e-6; the exact singular values are s."""
     n = 4096
     _say("case geometric n=%d b=%d %s" % (n, b, np.dtype(dt).name))
+    import torch
     t, s = inputs.geometric_matrix_device(n, BASE + 3, 1e6, dtype=dt)
+    # the handle's stream does not wait for torch's: the input must be complete
+    torch.cuda.synchronize()
     R = Reducer(n, b, "f64" if dt == np.float64 else "f32")
     R.reduce(t.data_ptr(), reps=1)
     del t

@@ -225,7 +225,8 @@ def test_svdvals_after_tiled_qr_is_refused(gpu):
     with pytest.raises(AvsError) as ei:
         T.svdvals()
     assert ei.value.status == 3  # AVS_ERR_STATE
-    T.reload()  # a fresh upload makes it usable again
+    T.A[:] = a  # tiled_QR! wrote R and the reflectors into T.A, as the reference does in place
+    T.reload()  # a fresh upload makes the handle usable again
     assert O.sigma_err(T.svdvals(), sl.svdvals(a)) <= 10 * n * np.finfo(np.float64).eps
 
 

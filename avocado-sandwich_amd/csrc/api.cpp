@@ -441,24 +441,17 @@ template <typename TS, int B> avs_status run_streams_graph(avs_handle *h)
 namespace avs {
 
 // Singular values of a bidiagonal by bisection on the device (one lane per
-// value) on stream s, with the caller's buffers: dT2 (2n doubles), dSig (n)
-// on the device and hT2, hSig (the same sizes, pinned) on the host; LAPACK
-// dbdsdc on the host for n < 128 (the device round trip costs more) or with
-// AVSVD_BDSDC=host.  Allocates nothing on the device path.
-avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma, double *dT2,
-                        double *dSig, double *hT2, double *hSig, void *sv)
+// value): enqueues on stream `sv` the upload of the Golub-Kahan tridiagonal's
+// squared off-diagonals (through the caller's pinned hT2, 2n doubles, to dT2),
+// the bisection into dSig (n doubles) and the download of the nkeep largest
+// into the caller's pinned hSig; valid once the stream reaches this point.
+// Allocates nothing.
+avs_status bidiag_sigma_enqueue(int64_t n, int64_t nkeep, const double *d, const double *e, double *dT2,
+                                double *dSig, double *hT2, double *hSig, void *sv)
 {
     hipStream_t s = (hipStream_t)sv;
     if (n <= 0 || nkeep <= 0 || nkeep > n) return fail(-1, "bad sizes n=%lld nkeep=%lld", (long long)n,
                                                        (long long)nkeep);
-    const char *pol = getenv("AVSVD_BDSDC");
-    if (n < 128 || (pol && strcmp(pol, "host") == 0)) {
-        std::vector<double> sg(n);
-        avs_status st = avs_bdsdc_values(n, d, e, sg.data());
-        if (st) return st;
-        memcpy(sigma, sg.data(), sizeof(double) * nkeep);
-        return AVS_OK;
-    }
     // squared off-diagonals of the Golub-Kahan tridiagonal, its Gershgorin
     // bound and LAPACK's pivmin
     double bnd = 0.0, tmx = 0.0, prev = 0.0;
@@ -476,9 +469,35 @@ avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double 
     launch_bisect(dT2, n, nkeep, hi0, pivmin, dSig, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(hSig, dSig, sizeof(double) * nkeep, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    return AVS_OK;
+}
+
+// The same, waited for: sigma gets the nkeep largest, descending.  LAPACK
+// dbdsdc on the host instead for n < 128 (the device round trip costs more)
+// or with AVSVD_BDSDC=host.
+avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma, double *dT2,
+                        double *dSig, double *hT2, double *hSig, void *sv)
+{
+    if (n <= 0 || nkeep <= 0 || nkeep > n) return fail(-1, "bad sizes n=%lld nkeep=%lld", (long long)n,
+                                                       (long long)nkeep);
+    if (!bisect_on_device(n)) {
+        std::vector<double> sg(n);
+        avs_status st = avs_bdsdc_values(n, d, e, sg.data());
+        if (st) return st;
+        memcpy(sigma, sg.data(), sizeof(double) * nkeep);
+        return AVS_OK;
+    }
+    avs_status st = bidiag_sigma_enqueue(n, nkeep, d, e, dT2, dSig, hT2, hSig, sv);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize((hipStream_t)sv));
     memcpy(sigma, hSig, sizeof(double) * nkeep);
     return AVS_OK;
+}
+
+bool bisect_on_device(int64_t n)
+{
+    const char *pol = getenv("AVSVD_BDSDC");
+    return n >= 128 && !(pol && strcmp(pol, "host") == 0);
 }
 
 // Band -> bidiagonal of one matrix on the device, from a band store (ld 3b+1,
@@ -518,6 +537,26 @@ avs_status chase_store_on_device(double *dStore, int64_t n, int b, int *dErr, do
     if (herr) return fail(AVS_ERR_STATE, "device chase: a band store access outside the band");
     return AVS_OK;
 }
+
+// Band of the handle's reduced matrix, FP64 (b+1) x n LAPACK upper-band
+// layout, into the handle's pinned host buffer (*host_band), enqueued on the
+// handle's stream: the buffer is valid once that stream has reached this
+// point.  Allocates the band buffers on first use only.
+avs_status band_enqueue(avs_handle *h, const double **host_band)
+{
+    if (h->state == avs_handle::ST_EMPTY) return fail(AVS_ERR_STATE, "no matrix set on the handle");
+    const size_t ld = (size_t)h->b + 1;
+    if (!h->dBand) HIPCHK(hipMalloc(&h->dBand, sizeof(double) * ld * h->n));
+    if (!h->hBand) HIPCHK(hipHostMalloc((void **)&h->hBand, sizeof(double) * ld * h->n, hipHostMallocDefault));
+    if (h->dtype == AVS_F64) launch_band<double>(h->dTiles, h->dBand, (long)ld, h->nt, h->b, h->stream);
+    else launch_band<float>(h->dTiles, h->dBand, (long)ld, h->nt, h->b, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->hBand, h->dBand, sizeof(double) * ld * h->n, hipMemcpyDeviceToHost, h->stream));
+    *host_band = h->hBand;
+    return AVS_OK;
+}
+
+void *handle_stream(avs_handle *h) { return h->stream; }
 
 } // namespace avs
 
@@ -885,17 +924,12 @@ avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband)
     if (!h) return fail(-1, "null handle");
     if (!band) return fail(-2, "null band");
     if (ldband < h->b + 1) return fail(-3, "ldband < b+1");
-    if (h->state == avs_handle::ST_EMPTY) return fail(AVS_ERR_STATE, "no matrix set on the handle");
-    if (!h->dBand) HIPCHK(hipMalloc(&h->dBand, sizeof(double) * (size_t)(h->b + 1) * h->n));
-    if (h->dtype == AVS_F64) launch_band<double>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
-    else launch_band<float>(h->dTiles, h->dBand, h->b + 1, h->nt, h->b, h->stream);
-    HIPCHK(hipGetLastError());
-    // band-only D2H through a pinned staging buffer, then into the caller's layout
-    const size_t ld = (size_t)h->b + 1;
-    if (!h->hBand) HIPCHK(hipHostMalloc((void **)&h->hBand, sizeof(double) * ld * h->n, hipHostMallocDefault));
-    HIPCHK(hipMemcpyAsync(h->hBand, h->dBand, sizeof(double) * ld * h->n, hipMemcpyDeviceToHost, h->stream));
+    const double *hb = nullptr;
+    avs_status st = avs::band_enqueue(h, &hb);
+    if (st) return st;
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (int64_t j = 0; j < h->n; ++j) memcpy(band + j * ldband, h->hBand + j * ld, sizeof(double) * ld);
+    const size_t ld = (size_t)h->b + 1;
+    for (int64_t j = 0; j < h->n; ++j) memcpy(band + j * ldband, hb + j * ld, sizeof(double) * ld);
     return check_status(h);
 }
 

@@ -21,6 +21,18 @@ avs_status bidiagonalize_band_auto(int64_t n, int32_t bw, const double *band, in
 // with AVSVD_BDSDC=host (bidiag_sigma in api.cpp)
 avs_status bidiag_sigma(int64_t n, int64_t nkeep, const double *d, const double *e, double *sigma, double *dT2,
                         double *dSig, double *hT2, double *hSig, void *s);
+// the device part of bidiag_sigma without the wait: the nkeep values land in
+// hSig once stream `s` reaches this point
+avs_status bidiag_sigma_enqueue(int64_t n, int64_t nkeep, const double *d, const double *e, double *dT2,
+                                double *dSig, double *hT2, double *hSig, void *s);
+// whether bidiag_sigma bisects on the device for order n (n >= 128, unless
+// AVSVD_BDSDC=host)
+bool bisect_on_device(int64_t n);
+// the handle's band, FP64 (b+1) x n upper-band layout, into its pinned host
+// buffer (*host_band), enqueued on the handle's stream (no wait)
+avs_status band_enqueue(avs_handle *h, const double **host_band);
+// the handle's stream (hipStream_t)
+void *handle_stream(avs_handle *h);
 // band -> bidiagonal by Householder bulge chasing, sweeps pipelined over
 // nthreads host threads (bandbrd.cpp); the result does not depend on nthreads
 avs_status bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,

@@ -1,20 +1,22 @@
 // server.cpp -- continuous-batching svdvals request loop.
 //
 // Requests (any n, FP32 or FP64) are accepted as they arrive and admitted to
-// the device first-come first-served.  The device state lives in a pool of
-// handles (tile-major working copy, upload staging, factor slots, band
-// buffers), one per (dtype, padded order N, tile size b); a handle is created
-// when a request of its shape is admitted and none is idle, kept after the
-// request and reused by the next request of the same shape, so a steady
-// stream of requests allocates nothing.  Every pooled handle counts against
-// the HBM budget, idle or busy; when a new shape does not fit, idle handles
-// of other shapes are destroyed (least recently used first).  Each of
-// `max_inflight` device workers runs one request at a time on its handle's
-// streams, so the band reductions of admitted requests overlap on the GPU.
-// Only the (b+1) x N band comes back over PCIe; the host stage (bulge chasing
-// in FP64, then the bidiagonal's singular values by bisection on the device
-// or LAPACK dbdsdc) runs on a pool of host workers, each with its own stream
-// and bisection buffers sized once for the largest order the budget admits.
+// the device first-come first-served.  One device thread makes every HIP
+// call of the loop: it admits requests, enqueues each one's upload, band
+// reduction and band download on the streams of a pooled handle, polls their
+// completion events, and enqueues the bisections of finished bidiagonals; up
+// to `max_inflight` reductions are in flight at once, so they overlap on the
+// GPU.  Host workers do CPU work only: band -> bidiagonal (bulge chasing in
+// FP64) and, unless the device bisection is selected, LAPACK dbdsdc.
+//
+// The device state lives in a pool of handles (tile-major working copy,
+// upload staging, factor slots, band buffers), one per (dtype, padded order
+// N, tile size b); a handle is created when a request of its shape is
+// admitted and none is idle, kept after the request and reused by the next
+// request of the same shape, so a steady stream of requests allocates
+// nothing.  Every pooled handle counts against the HBM budget, idle or busy;
+// when a new shape does not fit, idle handles of other shapes are destroyed
+// (least recently used first).  Only the (b+1) x N band comes back over PCIe.
 // Latency is arrival -> singular values ready.
 #include <hip/hip_runtime_api.h>
 
@@ -50,7 +52,7 @@ struct Request {
     int b = 32;
     size_t footprint = 0;
     std::vector<char> A;
-    std::vector<double> band, sigma;
+    std::vector<double> band, sigma, d, e;
     double t_arrive = 0, t_admit = 0, t_band = 0, t_done = 0;
     avs_status status = AVS_OK;
     std::string err;
@@ -100,9 +102,10 @@ size_t footprint_of(int dtype, int64_t N, int b)
 
 } // namespace
 
-// one pooled handle
+// one pooled handle and its completion event
 struct Slot {
     avs_handle *h = nullptr;
+    hipEvent_t ev = nullptr;
     int dtype = AVS_F64;
     int64_t N = 0;
     int b = 0;
@@ -111,11 +114,13 @@ struct Slot {
     uint64_t last_use = 0;
 };
 
-// per host worker: stream and bisection buffers (2*nmax and nmax doubles on
-// the device and pinned on the host), allocated at server creation
-struct HostBufs {
+// one in-flight bisection: stream, buffers sized for the largest order the
+// budget admits, completion event
+struct Bisect {
     hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
     double *dT2 = nullptr, *dSig = nullptr, *hT2 = nullptr, *hSig = nullptr;
+    std::shared_ptr<Request> r;
 };
 
 struct avs_server {
@@ -123,67 +128,49 @@ struct avs_server {
     std::vector<Slot> pool;
     uint64_t use_clock = 0;
     int64_t nmax = 0;
-    std::vector<HostBufs> hbufs;
     std::mutex mu;
     std::condition_variable cv_dev, cv_host, cv_done;
-    std::deque<std::shared_ptr<Request>> pending, host_q;
+    std::deque<std::shared_ptr<Request>> pending, host_q, sigma_q;
     std::map<uint64_t, std::shared_ptr<Request>> reqs;
     size_t used = 0;
     bool stop = false;
     uint64_t next_id = 1;
-    std::vector<std::thread> dev_workers, host_workers;
+    std::thread dev_thread;
+    std::vector<std::thread> host_workers;
     std::vector<double> lat;
     double t_first = -1, t_last = 0;
     int64_t completed = 0;
+    std::string dev_err; // a failure of the device thread's own set-up
 };
 
 namespace {
 
-void free_host_bufs(avs_server *S)
-{
-    for (HostBufs &hb : S->hbufs) {
-        if (hb.s) hipStreamSynchronize(hb.s);
-        hipFree(hb.dT2);
-        hipFree(hb.dSig);
-        if (hb.hT2) hipHostFree(hb.hT2);
-        if (hb.hSig) hipHostFree(hb.hSig);
-        if (hb.s) hipStreamDestroy(hb.s);
-        hb = HostBufs();
-    }
-}
-
 // Takes the request at the head of the queue once a pooled handle of its
 // shape is idle or the budget has room for a new one (after evicting idle
 // handles of other shapes, least recently used first).  Called with S->mu
-// held; returns the slot index or -1 (wait).  Handles to destroy are moved to
-// `evict` (destroyed by the caller without the lock).
-int admit(avs_server *S, const Request &r, std::vector<avs_handle *> &evict)
+// held; returns the slot's key (its last_use stamp) or 0 (wait).  Evicted
+// slots are moved to `evict`.
+uint64_t admit(avs_server *S, const Request &r, std::vector<Slot> &evict)
 {
-    int idle = -1;
-    for (size_t i = 0; i < S->pool.size(); ++i) {
-        const Slot &x = S->pool[i];
+    for (Slot &x : S->pool) {
         if (!x.busy && x.dtype == r.dtype && x.N == r.N && x.b == r.b) {
-            idle = (int)i;
-            break;
+            x.busy = true;
+            x.last_use = ++S->use_clock;
+            return x.last_use;
         }
-    }
-    if (idle >= 0) {
-        S->pool[idle].busy = true;
-        S->pool[idle].last_use = ++S->use_clock;
-        return idle;
     }
     const size_t budget = (size_t)S->opts.hbm_budget_bytes;
     size_t idle_bytes = 0;
     for (const Slot &x : S->pool)
         if (!x.busy) idle_bytes += x.bytes;
-    if (S->used - idle_bytes + r.footprint > budget) return -1; // wait for busy handles to finish
+    if (S->used - idle_bytes + r.footprint > budget) return 0; // wait for busy handles to finish
     while (S->used + r.footprint > budget) {
-        int lru = -1;
+        size_t lru = S->pool.size();
         for (size_t i = 0; i < S->pool.size(); ++i)
-            if (!S->pool[i].busy && (lru < 0 || S->pool[i].last_use < S->pool[lru].last_use)) lru = (int)i;
-        evict.push_back(S->pool[lru].h);
+            if (!S->pool[i].busy && (lru == S->pool.size() || S->pool[i].last_use < S->pool[lru].last_use)) lru = i;
+        evict.push_back(S->pool[lru]);
         S->used -= S->pool[lru].bytes;
-        S->pool.erase(S->pool.begin() + lru);
+        S->pool.erase(S->pool.begin() + (long)lru);
     }
     Slot x;
     x.dtype = r.dtype;
@@ -194,128 +181,274 @@ int admit(avs_server *S, const Request &r, std::vector<avs_handle *> &evict)
     x.last_use = ++S->use_clock;
     S->used += x.bytes;
     S->pool.push_back(x);
-    return (int)S->pool.size() - 1;
+    return x.last_use;
 }
 
-void device_worker(avs_server *S)
+Slot *slot_of(avs_server *S, uint64_t key)
 {
+    for (Slot &x : S->pool)
+        if (x.busy && x.last_use == key) return &x;
+    return nullptr;
+}
+
+void destroy_slot(Slot &x)
+{
+    if (x.h) avs_destroy(x.h);
+    if (x.ev) hipEventDestroy(x.ev);
+    x.h = nullptr;
+    x.ev = nullptr;
+}
+
+// marks a request done (with its status) and wakes its waiter
+void finish(avs_server *S, const std::shared_ptr<Request> &r)
+{
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->t_done = now_ms();
+        r->done = true;
+        if (!r->status) {
+            S->lat.push_back(r->t_done - r->t_arrive);
+            S->t_last = std::max(S->t_last, r->t_done);
+            S->completed++;
+        }
+    }
+    S->cv_done.notify_all();
+}
+
+// In flight on the device: the request, its slot key (slots are found by key:
+// only this thread changes the pool, but admit's push_back can move it), the
+// band's pinned host buffer
+struct Running {
+    std::shared_ptr<Request> r;
+    uint64_t key = 0;
+    const double *band = nullptr;
+};
+
+// Enqueues request r on slot x: handle (created on first use), upload, band
+// reduction, band download, completion event.  Device thread only.
+avs_status start(avs_server *S, Slot &x, Running &run)
+{
+    const Request &r = *run.r;
+    avs_status st = AVS_OK;
+    if (!x.h) {
+        avs_opts o;
+        avs_default_opts(&o);
+        o.update_mode = S->opts.update_mode;
+        if ((st = avs_create(&x.h, r.dtype, r.N, r.b, &o))) return st;
+        if (hipEventCreateWithFlags(&x.ev, hipEventDisableTiming) != hipSuccess)
+            return fail(AVS_ERR_HIP, "server: event creation failed");
+    }
+    if ((st = avs_set_matrix_host_sub(x.h, r.A.data(), r.n, r.n))) return st;
+    if ((st = avs_blockbidiag(x.h, 0))) return st;
+    if ((st = avs::band_enqueue(x.h, &run.band))) return st;
+    if (hipEventRecord(x.ev, (hipStream_t)avs::handle_stream(x.h)) != hipSuccess)
+        return fail(AVS_ERR_HIP, "server: event record failed");
+    return AVS_OK;
+}
+
+void device_thread(avs_server *S)
+{
+    std::vector<Running> running;
+    std::vector<Bisect> bis(S->opts.device_sigma ? std::max(1, (int)S->opts.max_inflight) : 0);
+    for (Bisect &q : bis) {
+        const size_t nm = (size_t)S->nmax;
+        if (hipStreamCreateWithFlags(&q.s, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&q.ev, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc(&q.dT2, sizeof(double) * 2 * nm) != hipSuccess ||
+            hipMalloc(&q.dSig, sizeof(double) * nm) != hipSuccess ||
+            hipHostMalloc((void **)&q.hT2, sizeof(double) * 2 * nm, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&q.hSig, sizeof(double) * nm, hipHostMallocDefault) != hipSuccess) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->dev_err = "server: bisection buffers could not be allocated";
+            S->opts.device_sigma = 0; // host workers fall back to dbdsdc
+            break;
+        }
+    }
     for (;;) {
-        std::shared_ptr<Request> r;
-        std::vector<avs_handle *> evict;
-        avs_handle *h = nullptr;
-        size_t key = 0; // slot identity: its position can move when others are evicted
+        std::vector<Slot> evict;
+        std::vector<Running> starting;
+        std::vector<std::shared_ptr<Request>> sig_new;
+        bool stop;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            int slot = -1;
-            S->cv_dev.wait(lk, [&] {
-                if (S->stop) return true;
-                if (S->pending.empty()) return false;
-                slot = admit(S, *S->pending.front(), evict);
-                return slot >= 0;
-            });
-            if (S->stop) {
-                lk.unlock();
-                for (avs_handle *e : evict) avs_destroy(e);
-                return;
+            // sleep while there is nothing to admit and nothing in flight
+            const bool busy_dev = !running.empty();
+            bool busy_bis = false;
+            for (const Bisect &q : bis) busy_bis |= (bool)q.r;
+            if (!busy_dev && !busy_bis && !S->stop && S->sigma_q.empty()) {
+                S->cv_dev.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->sigma_q.empty(); });
             }
-            r = S->pending.front();
-            S->pending.pop_front();
-            r->t_admit = now_ms();
-            h = S->pool[slot].h;
-            key = S->pool[slot].last_use;
-        }
-        for (avs_handle *e : evict) avs_destroy(e);
-        avs_status st = AVS_OK;
-        if (!h) {
-            avs_opts o;
-            avs_default_opts(&o);
-            o.update_mode = S->opts.update_mode;
-            st = avs_create(&h, r->dtype, r->N, r->b, &o);
-        }
-        if (!st) st = avs_set_matrix_host_sub(h, r->A.data(), r->n, r->n);
-        if (!st) st = avs_blockbidiag(h, 0);
-        if (!st) {
-            r->band.assign((size_t)(r->b + 1) * r->N, 0.0);
-            st = avs_get_band(h, r->band.data(), r->b + 1);
-        }
-        std::string msg = st ? avs_last_error() : "";
-        r->t_band = now_ms();
-        std::vector<char>().swap(r->A);
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            for (size_t i = 0; i < S->pool.size(); ++i) {
-                Slot &x = S->pool[i];
-                if (x.busy && x.last_use == key) {
-                    if (st) { // a failed handle is not reused
-                        if (h) evict.assign(1, h);
-                        S->used -= x.bytes;
-                        S->pool.erase(S->pool.begin() + i);
-                    } else {
-                        x.h = h;
-                        x.busy = false;
-                    }
-                    break;
+            stop = S->stop;
+            while (!stop && !S->pending.empty() && (int)(running.size() + starting.size()) < S->opts.max_inflight) {
+                const uint64_t key = admit(S, *S->pending.front(), evict);
+                if (!key) break;
+                Running run;
+                run.r = S->pending.front();
+                run.key = key;
+                run.r->t_admit = now_ms();
+                S->pending.pop_front();
+                starting.push_back(run);
+            }
+            for (Bisect &q : bis)
+                if (!q.r && !S->sigma_q.empty()) {
+                    q.r = S->sigma_q.front();
+                    S->sigma_q.pop_front();
+                    sig_new.push_back(q.r);
                 }
-            }
-            r->status = st;
-            r->err = msg;
-            S->host_q.push_back(r);
         }
-        if (st)
-            for (avs_handle *e : evict) avs_destroy(e);
-        S->cv_dev.notify_all();
-        S->cv_host.notify_one();
+        for (Slot &x : evict) destroy_slot(x);
+        if (stop) break;
+        for (Running &run : starting) {
+            Slot *x;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                x = slot_of(S, run.key);
+            }
+            avs_status st = start(S, *x, run);
+            if (st) {
+                run.r->status = st;
+                run.r->err = avs_last_error();
+                std::vector<Slot> bad;
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    Slot *y = slot_of(S, run.key);
+                    bad.push_back(*y); // a failed handle is not reused
+                    S->used -= y->bytes;
+                    S->pool.erase(S->pool.begin() + (y - S->pool.data()));
+                }
+                for (Slot &z : bad) destroy_slot(z);
+                std::vector<char>().swap(run.r->A);
+                finish(S, run.r);
+                continue;
+            }
+            running.push_back(run);
+        }
+        // new bisections
+        for (Bisect &q : bis) {
+            if (!q.r || std::find(sig_new.begin(), sig_new.end(), q.r) == sig_new.end()) continue;
+            Request &r = *q.r;
+            avs_status st = avs::bidiag_sigma_enqueue(r.N, r.N, r.d.data(), r.e.data(), q.dT2, q.dSig, q.hT2, q.hSig,
+                                                      q.s);
+            if (!st && hipEventRecord(q.ev, q.s) != hipSuccess) st = fail(AVS_ERR_HIP, "server: event record failed");
+            if (st) {
+                r.status = st;
+                r.err = avs_last_error();
+                std::shared_ptr<Request> done = q.r;
+                q.r.reset();
+                finish(S, done);
+            }
+        }
+        // poll the reductions in flight
+        bool progressed = !starting.empty() || !sig_new.empty();
+        for (size_t i = 0; i < running.size();) {
+            Running &run = running[i];
+            Slot *x;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                x = slot_of(S, run.key);
+            }
+            const hipError_t q = hipEventQuery(x->ev);
+            if (q == hipErrorNotReady) {
+                ++i;
+                continue;
+            }
+            Request &r = *run.r;
+            if (q != hipSuccess) {
+                r.status = AVS_ERR_HIP;
+                r.err = std::string("server: reduction failed: ") + hipGetErrorString(q);
+            } else {
+                r.band.assign(run.band, run.band + (size_t)(r.b + 1) * r.N);
+            }
+            r.t_band = now_ms();
+            std::vector<char>().swap(r.A);
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                Slot *y = slot_of(S, run.key);
+                y->busy = false;
+                if (!r.status) S->host_q.push_back(run.r);
+            }
+            if (r.status) finish(S, run.r);
+            else S->cv_host.notify_one();
+            running.erase(running.begin() + (long)i);
+            progressed = true;
+        }
+        // poll the bisections in flight
+        for (Bisect &q : bis) {
+            if (!q.r) continue;
+            const hipError_t e = hipEventQuery(q.ev);
+            if (e == hipErrorNotReady) continue;
+            Request &r = *q.r;
+            if (e != hipSuccess) {
+                r.status = AVS_ERR_HIP;
+                r.err = std::string("server: bisection failed: ") + hipGetErrorString(e);
+            } else {
+                r.sigma.assign(q.hSig, q.hSig + r.n); // drop the padding's zeros
+            }
+            std::shared_ptr<Request> done = q.r;
+            q.r.reset();
+            finish(S, done);
+            progressed = true;
+        }
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(100));
     }
+    // stop: wait for what is in flight, then release the pool and buffers
+    for (Running &run : running) {
+        Slot *x = slot_of(S, run.key);
+        if (x && x->ev) hipEventSynchronize(x->ev);
+    }
+    for (Bisect &q : bis) {
+        if (q.s) hipStreamSynchronize(q.s);
+        hipFree(q.dT2);
+        hipFree(q.dSig);
+        if (q.hT2) hipHostFree(q.hT2);
+        if (q.hSig) hipHostFree(q.hSig);
+        if (q.ev) hipEventDestroy(q.ev);
+        if (q.s) hipStreamDestroy(q.s);
+    }
+    for (Slot &x : S->pool) destroy_slot(x);
+    S->pool.clear();
 }
 
-void host_worker(avs_server *S, int wid)
+void host_worker(avs_server *S)
 {
-    const HostBufs &hb = S->hbufs[wid];
     for (;;) {
         std::shared_ptr<Request> r;
+        bool dev_sigma;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_host.wait(lk, [&] { return S->stop || !S->host_q.empty(); });
             if (S->host_q.empty()) return;
             r = S->host_q.front();
             S->host_q.pop_front();
+            dev_sigma = S->opts.device_sigma != 0;
         }
-        if (!r->status) {
-            const int64_t N = r->N;
-            std::vector<double> d(N), e(std::max<int64_t>(N - 1, 1)), sg(N);
-            // one host thread per request, four from n = 4096 (the largest
-            // requests set the p99; the sweeps pipeline well there)
-            avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, d.data(), e.data(),
-                                                        N >= 4096 ? 4 : 1);
-            // the bidiagonal's singular values: bisection on the device with
-            // this worker's stream and buffers (nothing allocated here), or
-            // LAPACK dbdsdc on this thread
-            if (!st) {
-                if (S->opts.device_sigma && N <= S->nmax)
-                    st = avs::bidiag_sigma(N, N, d.data(), e.data(), sg.data(), hb.dT2, hb.dSig, hb.hT2, hb.hSig,
-                                           hb.s);
-                else
-                    st = avs_bdsdc_values(N, d.data(), e.data(), sg.data());
-            }
-            if (st) {
-                r->status = st;
-                r->err = avs_last_error();
-            } else {
-                r->sigma.assign(sg.begin(), sg.begin() + r->n); // drop the padding's zeros
-            }
-        }
+        const int64_t N = r->N;
+        r->d.assign(N, 0.0);
+        r->e.assign(std::max<int64_t>(N - 1, 1), 0.0);
+        // one host thread per request, four from n = 4096 (the largest
+        // requests set the p99; the sweeps pipeline well there)
+        avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, r->d.data(), r->e.data(),
+                                                    N >= 4096 ? 4 : 1);
         std::vector<double>().swap(r->band);
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            r->t_done = now_ms();
-            r->done = true;
-            if (!r->status) {
-                S->lat.push_back(r->t_done - r->t_arrive);
-                S->t_last = std::max(S->t_last, r->t_done);
-                S->completed++;
+        if (!st && dev_sigma && N <= S->nmax && avs::bisect_on_device(N)) {
+            // the bisection runs on the device thread
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->sigma_q.push_back(r);
             }
+            S->cv_dev.notify_all();
+            continue;
         }
-        S->cv_done.notify_all();
+        if (!st) {
+            std::vector<double> sg(N);
+            st = avs_bdsdc_values(N, r->d.data(), r->e.data(), sg.data());
+            if (!st) r->sigma.assign(sg.begin(), sg.begin() + r->n); // drop the padding's zeros
+        }
+        if (st) {
+            r->status = st;
+            r->err = avs_last_error();
+        }
+        finish(S, r);
     }
 }
 
@@ -371,34 +504,18 @@ avs_status avs_server_create(avs_server **sp, const avs_server_opts *opts)
     if (opts) S->opts = *opts;
     else avs_server_default_opts(&S->opts);
     if (S->opts.hbm_budget_bytes <= 0) return fail(-2, "hbm_budget_bytes must be positive");
-    if (S->opts.max_inflight <= 0) S->opts.max_inflight = 4;
-    if (S->opts.host_threads <= 0) S->opts.host_threads = avs::host_thread_count();
     for (int32_t r : S->opts.reserved)
         if (r != 0) return fail(-2, "opts.reserved must be zero");
+    if (S->opts.max_inflight <= 0) S->opts.max_inflight = 4;
+    if (S->opts.host_threads <= 0) S->opts.host_threads = avs::host_thread_count();
     // the largest padded order the budget admits (FP32, the smallest
     // footprint per element), which sizes the bisection buffers
     int64_t nmax = 128;
     while (footprint_of(AVS_F32, 2 * nmax, 128) <= (size_t)S->opts.hbm_budget_bytes) nmax *= 2;
     while (footprint_of(AVS_F32, nmax + 128, 128) <= (size_t)S->opts.hbm_budget_bytes) nmax += 128;
     S->nmax = nmax;
-    if (S->opts.device_sigma) {
-        S->hbufs.resize(S->opts.host_threads);
-        for (HostBufs &hb : S->hbufs) {
-            hipError_t e = hipStreamCreateWithFlags(&hb.s, hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipMalloc(&hb.dT2, sizeof(double) * 2 * (size_t)nmax);
-            if (e == hipSuccess) e = hipMalloc(&hb.dSig, sizeof(double) * (size_t)nmax);
-            if (e == hipSuccess) e = hipHostMalloc((void **)&hb.hT2, sizeof(double) * 2 * (size_t)nmax, hipHostMallocDefault);
-            if (e == hipSuccess) e = hipHostMalloc((void **)&hb.hSig, sizeof(double) * (size_t)nmax, hipHostMallocDefault);
-            if (e != hipSuccess) {
-                free_host_bufs(S.get());
-                return fail(AVS_ERR_HIP, "server bisection buffers: %s", hipGetErrorString(e));
-            }
-        }
-    } else {
-        S->hbufs.resize(S->opts.host_threads);
-    }
-    for (int i = 0; i < S->opts.max_inflight; ++i) S->dev_workers.emplace_back(device_worker, S.get());
-    for (int i = 0; i < S->opts.host_threads; ++i) S->host_workers.emplace_back(host_worker, S.get(), i);
+    S->dev_thread = std::thread(device_thread, S.get());
+    for (int i = 0; i < S->opts.host_threads; ++i) S->host_workers.emplace_back(host_worker, S.get());
     *sp = S.release();
     return AVS_OK;
 }
@@ -494,11 +611,8 @@ avs_status avs_server_destroy(avs_server *S)
     }
     S->cv_dev.notify_all();
     S->cv_host.notify_all();
-    for (auto &t : S->dev_workers) t.join();
     for (auto &t : S->host_workers) t.join();
-    for (Slot &x : S->pool)
-        if (x.h) avs_destroy(x.h);
-    free_host_bufs(S);
+    S->dev_thread.join();
     delete S;
     return AVS_OK;
 }

@@ -69,7 +69,7 @@ def cpu_baseline(ns=(1024, 4096), b=64):
                       "dgemqrt per cached block, block loop on %d threads with one BLAS thread each) restated in C, "
                       "FP64, b=%d, Gaussian n=%s; value = band stage at n=%d; bulge chasing "
                       "(bulgechasing.jl:1-42), dbdsdc and host dgesdd timed beside it" %
-                      (threads, b, "/".join(str(x) for x in ns), max(ns), max(ns)),
+                      (threads, b, "/".join(str(x) for x in ns), max(ns)),
             "stages": stages, "host_cpus": os.cpu_count()}
 
 
@@ -286,7 +286,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(tuple(args.cpu_ns), 64)
+        try:
+            cpu = cpu_baseline(tuple(args.cpu_ns), 64)
+        except Exception as e:
+            cpu = {"error": repr(e)}
 
     out = {
         "metric": METRIC,

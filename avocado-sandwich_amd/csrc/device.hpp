@@ -63,9 +63,9 @@ __device__ __forceinline__ long toff(int r, int c, long ld, bool tr)
 }
 
 // Panel thread geometry for NT threads: the NT/64 waves split the B columns
-// evenly; the LPC lanes of one column (lanes l, l+CPW, l+2CPW, ... of its wave)
-// hold RPL consecutive rows each, so every column-wise reduction is a lane
-// butterfly inside a wave.
+// evenly; the LPC lanes of one column (LPC consecutive lanes of its wave)
+// hold RPL consecutive rows each, so every column-wise reduction is a DPP
+// butterfly inside a row of lanes.
 template <int B, int NT = NTHREADS> struct PanelGeo {
     static constexpr int CPW = B / (NT / 64); // columns per wave
     static constexpr int LPC = 64 / CPW;      // lanes per column
@@ -94,10 +94,12 @@ template <typename TS, int B> struct Lds {
 //   ^8  : row_ror:8 inside each row of 16 lanes
 //   ^16 : v_permlane16_swap (odd rows of the first operand <-> even rows of the second)
 //   ^32 : v_permlane32_swap (upper half of the first operand <-> lower half of the second)
+// (every control used here reads a lane inside the wave's row, so no lane
+// takes the bound_ctrl zero)
 template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 template <int M> __device__ __forceinline__ double lane_xor(double v)
@@ -137,6 +139,21 @@ template <int CPW> __device__ __forceinline__ double colsum(double v)
     if constexpr (CPW <= 8) v += lane_xor<8>(v);
     if constexpr (CPW <= 16) v += lane_xor<16>(v);
     v += lane_xor<32>(v);
+    return v;
+}
+
+// sum over a group of LPC consecutive lanes (LPC = 4, 8 or 16), identical in
+// every lane of the group, with DPP moves only: quad_perm [1,0,3,2] (lane ^ 1),
+// quad_perm [2,3,0,1] (lane ^ 2), then row_half_mirror (pairs the two quads
+// of 8 lanes once each quad holds its sum) and row_mirror (the two halves of
+// 16).  Each step adds the same two values in either order.
+template <int LPC> __device__ __forceinline__ double grpsum(double v)
+{
+    static_assert(LPC == 4 || LPC == 8 || LPC == 16, "grpsum: 4, 8 or 16 lanes");
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    if constexpr (LPC >= 8) v += dpp_mov<0x141>(v);
+    if constexpr (LPC >= 16) v += dpp_mov<0x140>(v);
     return v;
 }
 
@@ -246,7 +263,11 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
     const int l = t & 63, w = t >> 6;
+#ifdef AVS_PANEL_STRIDED
     const int c = w * CPW + (l % CPW), q = l / CPW, r0 = q * RPL;
+#else
+    const int c = w * CPW + l / LPC, q = l % LPC, r0 = q * RPL;
+#endif
 
     double *Rp = smem;                    // packed upper triangle of R (TS/TT); diagonal read as alpha only
     double *xbuf = Rp + B * (B + 1) / 2;  // 2 x XS: published column
@@ -304,10 +325,19 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
     if (MODE == MODE_GE) pass_row(0);
 
     const int jend = (MODE == MODE_GE) ? B - 1 : B;
-    for (int j = 0; j < jend; ++j) {
+    for (int jv = 0; jv < jend; ++jv) {
         __syncthreads(); // column j published, previous reflector's writes visible
+        // the reflector index in a scalar register: every index derived from
+        // it (packed R, broadcast buffers, Gram rows) is formed on the SALU
+        const int j = __builtin_amdgcn_readfirstlane(jv);
         const int j16 = j & ~15;
         if ((w + 1) * CPW <= j16) continue; // every column of this wave is finished
+#ifdef AVS_PANEL_PRIO
+        // the wave that publishes the next pivot column issues first
+        if (__builtin_amdgcn_readfirstlane(w) == __builtin_amdgcn_readfirstlane((j + 1) / CPW))
+            __builtin_amdgcn_s_setprio(3);
+        else __builtin_amdgcn_s_setprio(0);
+#endif
         AVS_PANEL_MARK(0, j, c == j + 1 && q == 0);
         const double *xb = xbuf + (j & 1) * XS;
         const double *nb = nbuf + (j & 1) * LPC;
@@ -346,7 +376,11 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
         const double scal = srn * itau;
         const double c2 = rn * rn * itau;
         // S_c = x . a_c
+#ifdef AVS_PANEL_STRIDED
         const double S = colsum<CPW>((Sp[0] + Sp[1]) + (Sp[2] + Sp[3]));
+#else
+        const double S = grpsum<LPC>((Sp[0] + Sp[1]) + (Sp[2] + Sp[3]));
+#endif
         AVS_PANEL_MARK(2, j, c == j + 1 && q == 0 && S == S && c2 == c2);
         if (n2 != 0.0) {
             // GE: w_c = scal (S - beta a_c[j]);  TS/TT: w_c = R[j][c] + scal S

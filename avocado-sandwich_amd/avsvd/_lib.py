@@ -24,7 +24,7 @@ EXPORTS = (
     "avs_get_matrix_host", "avs_get_matrix_device", "avs_blockbidiag", "avs_tiled_qr", "avs_get_band",
     "avs_svdvals", "avs_synchronize", "avs_last_time_ms", "avs_last_error", "avs_bidiagonalize_band",
     "avs_bidiagonalize_band_lapack", "avs_bidiagonalize_band_chase", "avs_bidiagonalize_band_device",
-    "avs_bdsdc_values", "avs_geqrt", "avs_tsqrt", "avs_unmqr", "avs_tsmqr", "avs_gen_gaussian_host",
+    "avs_bdsdc_values", "avs_geqrt", "avs_tsqrt", "avs_unmqr", "avs_tsmqr", "avs_tsmqr_noalloc", "avs_gen_gaussian_host",
     "avs_debug_mfma_layout", "avs_schedule_stats", "avs_schedule_dump", "avs_phase_dump", "avs_phase_timing", "avs_phase_trace",
     "avs_batch_create", "avs_batch_destroy", "avs_batch_set_matrix_host", "avs_batch_blockbidiag",
     "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals", "avs_set_matrix_host_sub",
@@ -98,6 +98,7 @@ def lib():
             "avs_tsqrt": [i32, i32, vp, i64, vp, i64, vp, i32, vp],
             "avs_unmqr": [i32, i32, vp, i64, vp, vp, i64, i64, i32, i32, i32, vp],
             "avs_tsmqr": [i32, i32, vp, i64, vp, i64, vp, i64, vp, i64, i32, i32, i32, vp],
+            "avs_tsmqr_noalloc": [i32, i32, vp, i64, vp, i64, vp, i64, vp, i64, i32, i32, i32, vp],
             "avs_gen_gaussian_host": [ctypes.c_uint64, i64, ctypes.c_double, i32, vp, i64],
             "avs_debug_mfma_layout": [i32, dp, ctypes.POINTER(ctypes.c_float)],
             "avs_schedule_stats": [i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(i64), dp],

@@ -1038,28 +1038,41 @@ avs_status avs_tsqrt(int32_t dtype, int32_t b, void *dA1, int64_t lda1, void *dA
     });
 }
 
-static avs_status apply_op(int mode, int32_t dtype, int32_t b, const void *dV, int64_t ldv, const double *dtau,
-                           void *dC1, int64_t ldc1, void *dC2, int64_t ldc2, int64_t ncols, int32_t applyt,
-                           int32_t vtrans, int32_t ctrans, void *stream)
+// T_JJ of the reflectors in dV into dT (16 * b doubles), then the update of
+// ncols / b column tiles; both launches on stream s
+static avs_status apply_op_ws(int mode, int32_t dtype, int32_t b, const void *dV, int64_t ldv, const double *dtau,
+                              void *dC1, int64_t ldc1, void *dC2, int64_t ldc2, int64_t ncols, int32_t applyt,
+                              int32_t vtrans, int32_t ctrans, double *dT, hipStream_t s)
 {
-    hipStream_t s = (hipStream_t)stream;
     const int ntl = (int)(ncols / b);
     return dispatch_tb(dtype, b, [&](auto tp, auto bc) -> avs_status {
         typedef std::remove_pointer_t<decltype(tp)> TS;
         constexpr int B = decltype(bc)::value;
-        double *dT = nullptr;
-        HIPCHK(hipMalloc(&dT, sizeof(double) * 16 * B));
         launch_tjj<TS, B>(mode, dV, ldv, vtrans, dtau, dT, s);
         const long s1 = ctrans ? (long)B : (long)B * ldc1;
         const long s2 = ctrans ? (long)B : (long)B * ldc2;
         launch_update<TS, B>(mode, applyt != 0, dV, ldv, vtrans, dT, dC1, ldc1, s1, dC2, ldc2, s2, ctrans, ntl,
                              upd_tiles_for(B), 0, s);
-        hipError_t e = hipGetLastError();
-        hipStreamSynchronize(s);
-        hipFree(dT);
+        const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(AVS_ERR_HIP, "update launch failed: %s", hipGetErrorString(e));
         return AVS_OK;
     });
+}
+
+static avs_status apply_op(int mode, int32_t dtype, int32_t b, const void *dV, int64_t ldv, const double *dtau,
+                           void *dC1, int64_t ldc1, void *dC2, int64_t ldc2, int64_t ncols, int32_t applyt,
+                           int32_t vtrans, int32_t ctrans, void *stream)
+{
+    const avs_status built = dispatch_tb(dtype, b, [](auto, auto) { return AVS_OK; });
+    if (built) return built;
+    hipStream_t s = (hipStream_t)stream;
+    double *dT = nullptr;
+    HIPCHK(hipMalloc(&dT, sizeof(double) * 16 * (size_t)b));
+    const avs_status st =
+        apply_op_ws(mode, dtype, b, dV, ldv, dtau, dC1, ldc1, dC2, ldc2, ncols, applyt, vtrans, ctrans, dT, s);
+    hipStreamSynchronize(s);
+    hipFree(dT);
+    return st;
 }
 
 avs_status avs_unmqr(int32_t dtype, int32_t b, const void *dV, int64_t ldv, const double *dtau, void *dC,
@@ -1085,6 +1098,30 @@ avs_status avs_tsmqr(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC
     if (!dtau) return fail(-9, "null tau");
     if (ncols < 0 || ncols % b != 0) return fail(-10, "ncols must be a multiple of b");
     return apply_op(MODE_TS, dtype, b, dV, ldv, dtau, dC1, ldc1, dC2, ldc2, ncols, applyt, vtrans, ctrans, stream);
+}
+
+avs_status avs_tsmqr_noalloc(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC2, int64_t ldc2,
+                             const void *dV, int64_t ldv, const double *dtau, int64_t ncols, int32_t applyt,
+                             int32_t vtrans, int32_t ctrans, void *stream)
+{
+    if (b < 1 || 16L * b > NOALLOC_SCRATCH_DOUBLES)
+        return fail(-2, "b = %d: the fixed T_JJ workspace holds %d doubles (16 * b for b <= 128)", b,
+                    NOALLOC_SCRATCH_DOUBLES);
+    if (!dC1) return fail(-3, "null C1");
+    if (!dC2) return fail(-5, "null C2");
+    if (!dV) return fail(-7, "null V");
+    if (ldv < b) return fail(-8, "ldv < b");
+    if (!dtau) return fail(-9, "null tau");
+    if (ncols < 0 || ncols % b != 0) return fail(-10, "ncols must be a multiple of b");
+    hipStream_t s = (hipStream_t)stream;
+    int device = -1;
+    if (s) HIPCHK(hipStreamGetDevice(s, &device));
+    else HIPCHK(hipGetDevice(&device));
+    double *ws = nullptr;
+    const hipError_t e = noalloc_scratch(device, &ws);
+    if (e != hipSuccess)
+        return fail(AVS_ERR_HIP, "device %d: address of the T_JJ workspace: %s", device, hipGetErrorString(e));
+    return apply_op_ws(MODE_TS, dtype, b, dV, ldv, dtau, dC1, ldc1, dC2, ldc2, ncols, applyt, vtrans, ctrans, ws, s);
 }
 
 avs_status avs_debug_mfma_layout(int32_t variant, double *out64, float *out32)

@@ -8,6 +8,12 @@ namespace avs {
 struct Task;
 struct ChainOp;
 
+// T_JJ workspace of the allocation-free TSMQR: 16 * b doubles for b <= 128
+constexpr int NOALLOC_SCRATCH_DOUBLES = 16 * 128;
+// device address of `device`'s copy of that workspace (noalloc.hip); looked up
+// on the first call for each device, with that device current
+hipError_t noalloc_scratch(int device, double **ptr);
+
 template <typename TS> void launch_pack(const void *A, long lda, void *tiles, int nt, int b, hipStream_t s);
 template <typename TS>
 void launch_unpack(const void *tiles, void *A, long lda, int nt, int b, int mode, int kend, hipStream_t s);

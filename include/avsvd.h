@@ -233,6 +233,21 @@ avs_status avs_unmqr(int32_t dtype, int32_t b, const void *dV, int64_t ldv, cons
 avs_status avs_tsmqr(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC2, int64_t ldc2,
                      const void *dV, int64_t ldv, const double *dtau, int64_t ncols,
                      int32_t applyt, int32_t vtrans, int32_t ctrans, void *stream);
+/* avs_tsmqr_noalloc: avs_tsmqr that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The T_JJ factor that avs_tsmqr builds in a buffer of its own goes to a
+ * fixed per-GPU workspace of 16 * 128 doubles inside the library, which each
+ * GPU holds until the process ends; its address is looked up on the first
+ * call on each device.  b beyond that workspace (b > 128) is refused with -2.
+ * Every call on one GPU shares the workspace: calls that overlap on one GPU
+ * (on different streams, or from host threads without ordering between them)
+ * race; calls ordered on one stream do not.  It does not synchronize: the
+ * result is ready when `stream` has passed this call.  The same kernels and
+ * results as avs_tsmqr; arguments and other errors as avs_tsmqr. */
+avs_status avs_tsmqr_noalloc(int32_t dtype, int32_t b, void *dC1, int64_t ldc1, void *dC2, int64_t ldc2,
+                             const void *dV, int64_t ldv, const double *dtau, int64_t ncols,
+                             int32_t applyt, int32_t vtrans, int32_t ctrans, void *stream);
 
 /* ---- seeded synthetic inputs (both sides of every comparison) ----
  * Philox4x32-10 + Box-Muller, element e of the column-major matrix from

@@ -45,6 +45,14 @@ def test_create_rejects_unsupported_shapes_like_the_reference():
         avsvd.compile_kernels(tilesize=48)
 
 
+def test_tsmqr_noalloc_refuses_b_beyond_its_workspace():
+    # the fixed T_JJ workspace holds 16 * 128 doubles; the check runs before any HIP call
+    L = lib()
+    nul = ctypes.c_void_p()
+    assert L.avs_tsmqr_noalloc(0, 256, nul, 256, nul, 256, nul, 256, nul, 256, 1, 0, 0, nul) == -2
+    assert b"workspace" in L.avs_last_error()
+
+
 def test_generator_matches_oracle_bitwise():
     for n, seed in ((5, 1), (64, 20261203), (129, 77)):
         np.testing.assert_array_equal(avsvd.gaussian_matrix(n, seed), O.gaussian_matrix(n, seed))

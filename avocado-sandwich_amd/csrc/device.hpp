@@ -290,14 +290,19 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
         a[s] = (MODE == MODE_TT && row > c) ? 0.0 : (double)src_c[(long)s * rs];
     }
     if (MODE != MODE_GE) {
-        for (int idx = t; idx < B * B; idx += NT) {
-            int r, cc;
-            if (tr) { cc = idx % B; r = idx / B; } else { r = idx % B; cc = idx / B; }
-            if (r <= cc) {
-                const double v = (double)top[toff(r, cc, ldt, tr)];
-                Rp[rpk(r, cc)] = v;
-                if (r == cc) dgL[r] = v;
-            }
+        // R from this lane's rows of column c of the top tile, loaded like the
+        // column itself: every load of the lane is in flight at once (the
+        // whole column is read; only its upper part is kept)
+        const long rt = tr ? ldt : 1;
+        TS *top_c = top + toff(r0, c, ldt, tr);
+        double rv[RPL];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) rv[s] = (double)top_c[(long)s * rt];
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+            const int row = r0 + s;
+            if (row <= c) Rp[rpk(row, c)] = rv[s];
+            if (row == c) dgL[c] = rv[s];
         }
     }
     for (int idx = t; idx < 16 * B; idx += NT) gramL[idx] = 0.0;
@@ -427,20 +432,45 @@ __device__ __forceinline__ void panel_body(TS *__restrict__ top, long ldt, TS *_
     for (int idx = t; idx < B; idx += NT) tau[idx] = tauL[idx];
     TS *src_w = src;
     asm volatile("" : "+s"(src_w));
-    TS *dst_c = src_w + toff(r0, c, lds, tr);
+    // the lane's column and rows again, from a fresh thread index: nothing of
+    // the prologue's addressing (the same expressions) is kept across the loop
+    int te = threadIdx.x;
+    asm volatile("" : "+v"(te));
+#ifdef AVS_PANEL_STRIDED
+    const int ce = (te >> 6) * CPW + ((te & 63) % CPW), r0e = ((te & 63) / CPW) * RPL;
+#else
+    const int ce = (te >> 6) * CPW + (te & 63) / LPC, r0e = ((te & 63) % LPC) * RPL;
+#endif
+    TS *dst_c = src_w + toff(r0e, ce, lds, tr);
+    if (MODE == MODE_GE) {
+        // every row of column c: R above the diagonal and the diagonal (up to
+        // column B-2, which has no reflector) from LDS, the reflector below
 #pragma unroll
-    for (int s = 0; s < RPL; ++s) {
-        const int row = r0 + s;
-        // GE: the R part (above the diagonal, and the diagonal up to column B-2) comes from LDS below
-        const bool reg = (MODE == MODE_GE) ? (row > c || (row == c && c == B - 1)) : (MODE != MODE_TT || row <= c);
-        if (reg) dst_c[(long)s * rs] = (TS)a[s];
-    }
-    for (int idx = t; idx < B * B; idx += NT) {
-        int r, cc;
-        if (tr) { cc = idx % B; r = idx / B; } else { r = idx % B; cc = idx / B; }
-        if (MODE == MODE_GE && r == B - 1 && cc == B - 1) continue; // no reflector for the last column
-        if (r < cc) top[toff(r, cc, ldt, tr)] = (TS)Rp[rpk(r, cc)];
-        else if (r == cc) top[toff(r, cc, ldt, tr)] = (TS)dgL[r];
+        for (int s = 0; s < RPL; ++s) {
+            const int row = r0e + s;
+            if (row > ce || (row == ce && ce == B - 1)) dst_c[(long)s * rs] = (TS)a[s];
+        }
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+            const int row = r0e + s;
+            if (row < ce || (row == ce && ce < B - 1)) dst_c[(long)s * rs] = (TS)(row < ce ? Rp[rpk(row, ce)] : dgL[ce]);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+            const int row = r0e + s;
+            if (MODE != MODE_TT || row <= ce) dst_c[(long)s * rs] = (TS)a[s];
+        }
+        // R: rows <= c of column c of the top tile; its lower part stays as it was
+        TS *top_w = top;
+        asm volatile("" : "+s"(top_w));
+        const long rt = tr ? ldt : 1;
+        TS *top_c = top_w + toff(r0e, ce, ldt, tr);
+#pragma unroll
+        for (int s = 0; s < RPL; ++s) {
+            const int row = r0e + s;
+            if (row <= ce) top_c[(long)s * rt] = (TS)(row < ce ? Rp[rpk(row, ce)] : dgL[ce]);
+        }
     }
 
     // diagonal blocks of the compact-WY factor, one row per lane:

@@ -130,28 +130,36 @@ def tiled_QR_(A, kend=None, sync=True):
     return A
 
 
-def _ooc(A, f, kswitch, mydims2d, tilesize, tilefactor):
-    # OOC_alg! (tiledalgos.jl:232-251), in-core branch: one GPU holds n <= ~190k
-    # FP64 in HBM, so the out-of-core streaming branch is not needed here.
-    T = TiledMatrix(A, tilesize, tilesize, tiledim2d=int(tilesize // tilefactor) if mydims2d else 0)
-    f(T)
-    return T
+def _ooc(entry, A, kswitch, mydims2d, tilesize, tilefactor, opts):
+    """OOC_alg! (tiledalgos.jl:232-251) on the host matrix A, in place: the
+    first n/b - kswitch steps stream A through the GPU (LargeTiledMatrix,
+    TiledMatrix.jl:121-186, tiledalgos.jl:75-225), the trailing kswitch tiles
+    run in core; n <= kswitch * b runs in core.  mydims2d / tilefactor select
+    the reference's launch shape and are accepted for the signature."""
+    if A.ndim != 2 or A.shape[0] != A.shape[1] or A.shape[0] % tilesize:
+        raise AvsError(-2, "Not supported")
+    if not A.flags.f_contiguous:
+        raise AvsError(-4, "A must be a column-major (Fortran-ordered) array")
+    o = _opts(**opts)
+    check(entry(dtype_code(A.dtype), A.shape[0], tilesize, vptr(A), A.shape[0], int(kswitch), ctypes.byref(o)))
 
 
-def OOC_QR_(A, kswitch=10, mydims2d=False, tilesize=32, tilefactor=1):
-    """OOC_QR! (tiledalgos.jl:253): tiled QR of the host matrix A in place."""
-    _ooc(A, tiled_QR_, kswitch, mydims2d, tilesize, tilefactor)
+def OOC_QR_(A, kswitch=10, mydims2d=False, tilesize=32, tilefactor=1, **opts):
+    """OOC_QR! (tiledalgos.jl:253): tiled QR of the host matrix A in place;
+    returns nothing, like the reference."""
+    _ooc(lib().avs_ooc_tiled_qr, A, kswitch, mydims2d, tilesize, tilefactor, opts)
 
 
-def OOC_Bidiag_(A, kswitch=10, mydims2d=False, tilesize=32, tilefactor=1):
-    """OOC_Bidiag! (tiledalgos.jl:254): band reduction of host A in place."""
-    _ooc(A, Blockbidiag_, kswitch, mydims2d, tilesize, tilefactor)
+def OOC_Bidiag_(A, kswitch=10, mydims2d=False, tilesize=32, tilefactor=1, **opts):
+    """OOC_Bidiag! (tiledalgos.jl:254): band reduction of the host matrix A in
+    place; returns nothing, like the reference."""
+    _ooc(lib().avs_ooc_blockbidiag, A, kswitch, mydims2d, tilesize, tilefactor, opts)
 
 
-def OOC_SVD_(A, kswitch=10, mydims2d=False, tilesize=32, tilefactor=1):
+def OOC_SVD_(A, kswitch=10, mydims2d=False, tilesize=32, tilefactor=1, **opts):
     """OOC_SVD! (tiledalgos.jl:256-260): band reduction, then bidiagonalize;
     returns the bidiagonal matrix like the reference (diag_lapack gives sigma)."""
-    OOC_Bidiag_(A, kswitch, mydims2d, tilesize, tilefactor)
+    OOC_Bidiag_(A, kswitch, mydims2d, tilesize, tilefactor, **opts)
     return bidiagonalize(A, tilesize)
 
 

@@ -167,7 +167,6 @@ int resolve_tree(const avs_handle *h) { return h->opts.tree; }
 // shortest for m >= 4 up to one, and the update work is the same for every
 // domain size)
 int resolve_domain(const avs_handle *h) { return h->opts.domain > 0 ? h->opts.domain : 0; }
-int upd_tiles_for(int b) { return b >= 128 ? 1 : (b == 64 ? 2 : 4); }
 
 avs_status ensure_program(avs_handle *h, int alg, int kend)
 {

@@ -39,4 +39,6 @@ avs_status bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, i
                                     int nthreads);
 // host threads the library uses (AVSVD_HOST_THREADS, else min(16, cores))
 int host_thread_count();
+// column tiles one update workgroup takes (its 8 strips of 16 columns)
+inline int upd_tiles_for(int b) { return b >= 128 ? 1 : (b == 64 ? 2 : 4); }
 } // namespace avs

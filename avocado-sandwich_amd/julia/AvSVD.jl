@@ -4,7 +4,7 @@
 # Drop-in for (paths relative to the reference repository root):
 #   KAbasedSVD/src/TiledMatrix.jl:41-67      TiledMatrix(A, bm, bn; tiledim2d)
 #   KAbasedSVD/src/tiledalgos.jl:6-34        tiled_QR!(A; kend), Blockbidiag!(A; kend)
-#   KAbasedSVD/src/tiledalgos.jl:232-267     OOC_QR!/OOC_Bidiag!/OOC_SVD!, compile_kernels
+#   KAbasedSVD/src/tiledalgos.jl:232-267     OOC_QR!/OOC_Bidiag!/OOC_SVD! (streamed), compile_kernels
 #   KAbasedSVD/src/bulgechasing.jl:33-53     bidiagonalize, my_CU_svdval, diag_lapack
 # A caller replaces
 #   include("QRkernels.jl"); include("QRkernelscompile.jl"); include("TiledMatrix.jl");
@@ -47,7 +47,8 @@ end
 function check(st::Integer)
     if st != 0
         msg = unsafe_string(ccall((:avs_last_error, libavsvd), Cstring, ()))
-        st == -3 || st == -4 ? error("Not supported") : error("avsvd status $st: $msg")
+        # the reference's shape errors keep their text
+        occursin("Not supported", msg) ? error("Not supported") : error("avsvd status $st: $msg")
     end
     return nothing
 end
@@ -70,9 +71,10 @@ end
 tune_set!(::Type{T}, n_upto::Integer, b::Integer) where {T<:Union{Float32,Float64}} =
     check(ccall((:avs_tune_set, libavsvd), Int32, (Int32, Int64, Int32), dtype(T), n_upto, b))
 
-mutable struct TiledMatrix{T<:Union{Float32,Float64}}
+mutable struct TiledMatrix{T<:Union{Float32,Float64},AT<:AbstractMatrix{T}}
     handle::Ptr{Cvoid}
-    A::Matrix{T}
+    A::AT          # the caller's array, host Matrix or device array; results go back into it
+    device::Bool   # A lives in GPU memory (pointer(A) is a device address)
     M::Int
     N::Int
     m::Int
@@ -80,23 +82,47 @@ mutable struct TiledMatrix{T<:Union{Float32,Float64}}
     no_tiles::Int
 end
 
-# TiledMatrix(A, blocksize_m, blocksize_n; tiledim2d)  (TiledMatrix.jl:41-67)
-function TiledMatrix(A::Matrix{T}, blocksize_m::Int, blocksize_n::Int; tiledim2d::Int=0,
+# TiledMatrix(A, blocksize_m, blocksize_n; tiledim2d)  (TiledMatrix.jl:41-67).
+# A host Matrix is uploaded with avs_set_matrix_host.  Any other dense
+# column-major array whose pointer(A) is a GPU address (a ROCArray{T,2} from
+# AMDGPU.jl) is read in place on the device with avs_set_matrix_device, and the
+# algorithms write their result back into that same device array, as the
+# reference's TiledMatrix aliases its CuArray.  The copy runs on the handle's
+# stream: work still queued on the array (another stream) must be finished
+# first (AMDGPU.synchronize()), or opts.stream set to the caller's stream.
+function TiledMatrix(A::AbstractMatrix{T}, blocksize_m::Int, blocksize_n::Int; tiledim2d::Int=0,
                      opts::AvsOpts=default_opts()) where {T<:Union{Float32,Float64}}
     if (blocksize_m != blocksize_n) || size(A, 1) != size(A, 2) || mod(size(A, 1), blocksize_n) != 0
         error("Not supported")
     end
+    stride(A, 1) == 1 || error("A must be column-major with unit row stride")
+    device = !(A isa Array)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:avs_create, libavsvd), Int32, (Ref{Ptr{Cvoid}}, Int32, Int64, Int32, Ref{AvsOpts}),
                 h, dtype(T), size(A, 1), blocksize_m, opts))
-    check(ccall((:avs_set_matrix_host, libavsvd), Int32, (Ptr{Cvoid}, Ptr{T}, Int64), h[], A, stride(A, 2)))
-    M = TiledMatrix{T}(h[], A, size(A, 1), size(A, 2), blocksize_m, blocksize_n, div(size(A, 1), blocksize_n))
+    sym = device ? :avs_set_matrix_device : :avs_set_matrix_host
+    st = ccall((sym, libavsvd), Int32, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), h[], Ptr{Cvoid}(pointer(A)), stride(A, 2))
+    if st != 0
+        ccall((:avs_destroy, libavsvd), Int32, (Ptr{Cvoid},), h[])
+        check(st)
+    end
+    M = TiledMatrix{T,typeof(A)}(h[], A, device, size(A, 1), size(A, 2), blocksize_m, blocksize_n,
+                                 div(size(A, 1), blocksize_n))
     finalizer(x -> ccall((:avs_destroy, libavsvd), Int32, (Ptr{Cvoid},), x.handle), M)
     return M
 end
 
+# the result back into the caller's array (host: avs_get_matrix_host; device:
+# avs_get_matrix_device on the handle's stream, then a wait for it)
 function sync_to_host!(A::TiledMatrix{T}) where {T}
-    check(ccall((:avs_get_matrix_host, libavsvd), Int32, (Ptr{Cvoid}, Ptr{T}, Int64), A.handle, A.A, stride(A.A, 2)))
+    if A.device
+        check(ccall((:avs_get_matrix_device, libavsvd), Int32, (Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                    A.handle, Ptr{Cvoid}(pointer(A.A)), stride(A.A, 2)))
+        check(ccall((:avs_synchronize, libavsvd), Int32, (Ptr{Cvoid},), A.handle))
+    else
+        check(ccall((:avs_get_matrix_host, libavsvd), Int32, (Ptr{Cvoid}, Ptr{T}, Int64), A.handle, A.A,
+                    stride(A.A, 2)))
+    end
     return A
 end
 
@@ -112,13 +138,20 @@ function Blockbidiag!(A::TiledMatrix; kend=A.no_tiles)
     return sync_to_host!(A)
 end
 
-# OOC_alg! in-core branch (tiledalgos.jl:232-251); one GPU's HBM holds the matrix
-function OOC_alg!(A::Matrix, f::Function; kswitch=10, mydims2d=false, tilesize=32, tilefactor=1, backend=nothing)
-    f(TiledMatrix(A, tilesize, tilesize; tiledim2d=mydims2d ? div(tilesize, tilefactor) : 0))
+# OOC_alg!  (tiledalgos.jl:232-251) on a host Matrix, in place: the first
+# n/b - kswitch steps stream A through the GPU one tile row or tile column at a
+# time (LargeTiledMatrix, TiledMatrix.jl:121-186, tiledalgos.jl:75-225), the
+# trailing kswitch tiles run in core; n <= kswitch * tilesize runs in core.
+# mydims2d / tilefactor / backend are accepted for the signature.
+function OOC_alg!(A::Matrix{T}, sym::Symbol; kswitch=10, mydims2d=false, tilesize=32, tilefactor=1,
+                  backend=nothing, opts::AvsOpts=default_opts()) where {T<:Union{Float32,Float64}}
+    size(A, 1) == size(A, 2) || error("Not supported")
+    check(ccall((sym, libavsvd), Int32, (Int32, Int64, Int32, Ptr{T}, Int64, Int32, Ref{AvsOpts}),
+                dtype(T), size(A, 1), tilesize, A, stride(A, 2), kswitch, opts))
     return nothing
 end
-OOC_QR!(A::Matrix; kw...) = OOC_alg!(A, tiled_QR!; kw...)
-OOC_Bidiag!(A::Matrix; kw...) = OOC_alg!(A, Blockbidiag!; kw...)
+OOC_QR!(A::Matrix; kw...) = OOC_alg!(A, :avs_ooc_tiled_qr; kw...)
+OOC_Bidiag!(A::Matrix; kw...) = OOC_alg!(A, :avs_ooc_blockbidiag; kw...)
 
 # OOC_SVD!  (tiledalgos.jl:256-260): returns the bidiagonal matrix like the reference
 function OOC_SVD!(A::Matrix; kswitch=10, mydims2d=false, tilesize=32, tilefactor=1, backend=nothing)

@@ -130,6 +130,24 @@ avs_status avs_blockbidiag(avs_handle *h, int32_t kend);
 /* tiled_QR!(A; kend). */
 avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
 
+/* ---- out-of-core runs on a host matrix (OOC_alg!, KAbasedSVD/src/tiledalgos.jl:232-251) ----
+ * avs_ooc_blockbidiag is OOC_Bidiag! (:254), avs_ooc_tiled_qr is OOC_QR! (:253): the
+ * column-major n x n host matrix A (leading dimension lda) is reduced in place.  When
+ * n > kswitch * b the first n/b - kswitch steps stream A through the GPU as the reference's
+ * LargeTiledMatrix does (KAbasedSVD/src/TiledMatrix.jl:121-186, tiledalgos.jl:75-225): the
+ * flat tree, one tile row (QR sweep) or tile column (LQ sweep) at a time against the
+ * sweep's pivot row or column, so the device holds 3 (n/b - k) tiles in step k; the
+ * trailing kswitch x kswitch tiles then run in core with `opts` (NULL = defaults).
+ * Otherwise the whole matrix runs in core.  The result is what avs_get_matrix_host returns
+ * after the same algorithm in core (the band with exact zeros outside it, or the raw R and
+ * reflectors of tiled_QR!); with opts->tree = AVS_TREE_FLAT it is the same bits.
+ * Errors: -1 dtype, -2 n not a positive multiple of b ("Not supported"), -3 b not built,
+ * -4 null A, -5 lda < n, -6 kswitch < 0; device errors as avs_blockbidiag. */
+avs_status avs_ooc_blockbidiag(int32_t dtype, int64_t n, int32_t b, void *A, int64_t lda, int32_t kswitch,
+                               const avs_opts *opts);
+avs_status avs_ooc_tiled_qr(int32_t dtype, int64_t n, int32_t b, void *A, int64_t lda, int32_t kswitch,
+                            const avs_opts *opts);
+
 /* Per-launch device time of the last avs_blockbidiag / avs_tiled_qr run under
  * AVS_SCHED_STREAMS, for handles created with opts.phase_timing != 0 (one event
  * pair per launch, on the stream the launch runs on; with the default second

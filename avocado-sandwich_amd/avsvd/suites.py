@@ -5,6 +5,7 @@
     python -m avsvd.suites precision [--n 16384] [--conds 1e2 1e4 1e6]
     python -m avsvd.suites batched   [--count 4096] [--plain-sample 128]
     python -m avsvd.suites serving   [--requests 48] [--budget-gb 64]
+    python -m avsvd.suites ooc       [--n 16384] [--b 128] [--kswitches 32 64]
 
 Every case prints one JSON line as soon as it finishes.  Inputs are synthetic
 and seeded (Philox4x32-10 + Box-Muller from the library, seed 20261203 + case
@@ -392,6 +393,55 @@ def suite_serving(args):
     emit(d)
 
 
+def _streamed_bytes(n, b, ksteps, es):
+    """Host<->device bytes of the streamed steps: every sweep moves its pivot
+    and each of its units (nt - k tiles) up and down once."""
+    nt = n // b
+    tiles = 0
+    for k in range(ksteps):
+        tiles += (nt - k) * (nt - k)          # QR sweep: tile rows k..nt-1
+        if k + 1 < nt:
+            tiles += (nt - k - 1) * (nt - k)  # LQ sweep: tile columns k+1..nt-1
+    return 2 * tiles * b * b * es
+
+
+def suite_ooc(args):
+    """OOC_Bidiag! (tiledalgos.jl:232-254) on a host Gaussian matrix against
+    the in-core run with a host matrix (upload, reduction, download), and the
+    singular values of the streamed result against the cached FP64 LAPACK
+    vector of the same matrix."""
+    from .tiled import OOC_Bidiag_, TiledMatrix, Blockbidiag_, _dense_to_band, bdsdc_values, bidiagonalize_band_chase
+    n, b = args.n, args.b
+    A0 = np.asfortranarray(gaussian_matrix(n, SEED, dtype=np.float64))
+    path = os.path.join(GOLDEN, "sigma_gauss_n%d_s%d.npy" % (n, SEED))
+    ref = np.load(path) if os.path.exists(path) else None
+    A = A0.copy(order="F")
+    t0 = time.perf_counter()
+    T = TiledMatrix(A, b)
+    Blockbidiag_(T)
+    del T
+    t_in = time.perf_counter() - t0
+    progress("ooc: in-core with host matrix %.2f s" % t_in)
+    for ks in args.kswitches:
+        A = A0.copy(order="F")
+        t0 = time.perf_counter()
+        OOC_Bidiag_(A, kswitch=ks, tilesize=b)
+        t_ooc = time.perf_counter() - t0
+        ksteps = max(0, n // b - ks)
+        moved = _streamed_bytes(n, b, ksteps, 8)
+        d = {"suite": "ooc", "n": n, "b": b, "dtype": "f64", "kswitch": ks, "streamed_steps": ksteps,
+             "ooc_s": round(t_ooc, 3), "in_core_host_matrix_s": round(t_in, 3),
+             "streamed_bytes": moved, "streamed_gb_per_s": round(moved / t_ooc / 1e9, 2),
+             "tflops": round(8.0 / 3.0 * n ** 3 / t_ooc / 1e12, 3),
+             "device_tiles_held_streaming": 3 * (n // b)}
+        if ref is not None:
+            band = _dense_to_band(A, b)
+            dd, ee = bidiagonalize_band_chase(band, n, b)
+            d["sigma_err_over_sigma_max"] = sigma_err(bdsdc_values(dd, ee), ref)
+            d["reference"] = "FP64 LAPACK dgesdd of the same matrix (cached)"
+        emit(d)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="suite", required=True)
@@ -418,9 +468,13 @@ def main(argv=None):
     p.add_argument("--inflight", type=int, default=4)
     p.add_argument("--device-sigma", action="store_true",
                    help="bidiagonals' singular values by bisection on the device in the serving loop")
+    p = sub.add_parser("ooc")
+    p.add_argument("--n", type=int, default=16384)
+    p.add_argument("--b", type=int, default=128)
+    p.add_argument("--kswitches", type=int, nargs="+", default=[32, 64])
     args = ap.parse_args(argv)
     {"plain": suite_plain, "stage2": suite_stage2, "precision": suite_precision, "batched": suite_batched,
-     "serving": suite_serving}[args.suite](args)
+     "serving": suite_serving, "ooc": suite_ooc}[args.suite](args)
 
 
 if __name__ == "__main__":

@@ -20,7 +20,11 @@
 // executors, so the result equals the in-core AVS_TREE_FLAT run bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "avsvd.h"
@@ -37,6 +41,72 @@ namespace {
         hipError_t e_ = (call);                                                                                        \
         if (e_ != hipSuccess) return fail(AVS_ERR_HIP, "%s failed: %s (%d)", #call, hipGetErrorString(e_), (int)e_);  \
     } while (0)
+
+// Host threads for packing and unpacking units: run(count, f) calls f(lo, hi)
+// on nth disjoint parts of [0, count), part 0 on the calling thread, and
+// returns when every part is done.  The threads live for one streamed run.
+class Pool {
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int, int)> *job_ = nullptr;
+    int count_ = 0, gen_ = 0, pending_ = 0, nth_;
+    bool stop_ = false;
+
+    void part(int id) const
+    {
+        const int lo = (int)((long)count_ * id / nth_), hi = (int)((long)count_ * (id + 1) / nth_);
+        if (lo < hi) (*job_)(lo, hi);
+    }
+    void loop(int id)
+    {
+        int seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            part(id);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+
+  public:
+    explicit Pool(int n) : nth_(n < 1 ? 1 : n)
+    {
+        for (int i = 1; i < nth_; ++i) th_.emplace_back([this, i] { loop(i); });
+    }
+    ~Pool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(int count, const std::function<void(int, int)> &f)
+    {
+        if (nth_ == 1 || count < 2 * nth_) {
+            f(0, count);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &f;
+            count_ = count;
+            pending_ = nth_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+};
 
 // device and pinned buffers of one streamed run; released on every exit path
 struct OocRes {
@@ -74,28 +144,29 @@ template <typename TS, int B> struct Ooc {
     TS *A;
     long lda;
     int nt, split;
+    Pool *pool;
     OocRes r;
     static constexpr long TSZ = (long)B * B;
 
     // unit u of a sweep: tile row u (QR: tiles (u, k..nt-1)) or tile column u
     // (LQ: tiles (k..nt-1, u)), as L = nt - k consecutive b x b column-major
     // tiles
-    void pack(bool lq, int k, int u, TS *dst) const
+    // (column c of tile t): B contiguous elements on both sides
+    void copy_unit(bool lq, int k, int u, TS *tiles, bool to_tiles)
     {
-        for (int t = 0; t < nt - k; ++t) {
-            const long r0 = lq ? (long)(k + t) * B : (long)u * B;
-            const long c0 = lq ? (long)u * B : (long)(k + t) * B;
-            for (int c = 0; c < B; ++c) memcpy(dst + t * TSZ + (long)c * B, A + r0 + (c0 + c) * lda, sizeof(TS) * B);
-        }
+        pool->run((nt - k) * B, [&](int lo, int hi) {
+            for (int x = lo; x < hi; ++x) {
+                const int t = x / B, c = x % B;
+                const long r0 = lq ? (long)(k + t) * B : (long)u * B;
+                const long c0 = lq ? (long)u * B : (long)(k + t) * B;
+                TS *h = A + r0 + (c0 + c) * lda, *d = tiles + t * TSZ + (long)c * B;
+                if (to_tiles) memcpy(d, h, sizeof(TS) * B);
+                else memcpy(h, d, sizeof(TS) * B);
+            }
+        });
     }
-    void unpack(bool lq, int k, int u, const TS *src) const
-    {
-        for (int t = 0; t < nt - k; ++t) {
-            const long r0 = lq ? (long)(k + t) * B : (long)u * B;
-            const long c0 = lq ? (long)u * B : (long)(k + t) * B;
-            for (int c = 0; c < B; ++c) memcpy(A + r0 + (c0 + c) * lda, src + t * TSZ + (long)c * B, sizeof(TS) * B);
-        }
-    }
+    void pack(bool lq, int k, int u, TS *dst) { copy_unit(lq, k, u, dst, true); }
+    void unpack(bool lq, int k, int u, TS *src) { copy_unit(lq, k, u, src, false); }
 
     avs_status setup()
     {
@@ -177,32 +248,33 @@ template <typename TS, int B> struct Ooc {
             // unit i-1 back on the host frees slot s^1 for unit i+1
             if (i > 0) {
                 OOCCHK(hipEventSynchronize(r.evDown[s ^ 1]));
-                unpack(lq, k, u0 + i - 1, (const TS *)r.hU[s ^ 1]);
+                unpack(lq, k, u0 + i - 1, (TS *)r.hU[s ^ 1]);
             }
             if (i + 1 < nu && (st = upload(i + 1))) return st;
         }
         if (nu > 0) {
             const int s = (nu - 1) & 1;
             OOCCHK(hipEventSynchronize(r.evDown[s]));
-            unpack(lq, k, u0 + nu - 1, (const TS *)r.hU[s]);
+            unpack(lq, k, u0 + nu - 1, (TS *)r.hU[s]);
         }
         // the pivot comes back after its last update
         OOCCHK(hipEventRecord(r.evP, r.cs));
         OOCCHK(hipStreamWaitEvent(r.down, r.evP, 0));
         OOCCHK(hipMemcpyAsync(r.hP, r.dP, lb, hipMemcpyDeviceToHost, r.down));
         OOCCHK(hipStreamSynchronize(r.down));
-        unpack(lq, k, pu, (const TS *)r.hP);
+        unpack(lq, k, pu, (TS *)r.hP);
         return AVS_OK;
     }
 
     // the zeros Blockbidiag! leaves in its first ksteps steps (tiledalgos.jl:
     // 46,58): outside the diagonal and super-diagonal tiles, and the strict
     // lower / strict upper part of those, in every tile row or column < ksteps
-    void band_rule(int ksteps) const
+    void band_rule(int ksteps)
     {
-        for (int j = 0; j < nt; ++j) {
-            for (int c = 0; c < B; ++c) {
-                TS *col = A + ((long)j * B + c) * lda;
+        pool->run(nt * B, [&](int lo, int hi) {
+            for (int x = lo; x < hi; ++x) {
+                const int j = x / B, c = x % B;
+                TS *col = A + (long)x * lda;
                 for (int i = 0; i < nt; ++i) {
                     if (i >= ksteps && j >= ksteps) continue;
                     TS *t = col + (long)i * B;
@@ -215,7 +287,7 @@ template <typename TS, int B> struct Ooc {
                     }
                 }
             }
-        }
+        });
     }
 
     avs_status run(int alg, int ksteps)
@@ -265,7 +337,8 @@ avs_status ooc_alg(int alg, int32_t dtype, int64_t n, int32_t b, void *A, int64_
     auto go = [&](auto tp, auto bc) {
         typedef std::remove_pointer_t<decltype(tp)> TS;
         constexpr int B = decltype(bc)::value;
-        Ooc<TS, B> w{(TS *)A, (long)lda, nt, split, {}};
+        Pool pool(host_thread_count());
+        Ooc<TS, B> w{(TS *)A, (long)lda, nt, split, &pool, {}};
         st = w.run(alg, ksteps);
     };
     if (dtype == AVS_F64) {

@@ -30,7 +30,7 @@ EXPORTS = (
     "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals", "avs_set_matrix_host_sub",
     "avs_server_default_opts", "avs_server_create", "avs_server_submit", "avs_server_poll", "avs_server_wait",
     "avs_server_stats", "avs_server_destroy", "avs_tune_set", "avs_tune_clear", "avs_tune_lookup",
-    "avs_device_arch", "avs_ooc_blockbidiag", "avs_ooc_tiled_qr",
+    "avs_device_arch", "avs_ooc_blockbidiag", "avs_ooc_tiled_qr", "avs_apply_q",
 )
 
 
@@ -101,6 +101,7 @@ def lib():
             "avs_tsmqr_noalloc": [i32, i32, vp, i64, vp, i64, vp, i64, vp, i64, i32, i32, i32, vp],
             "avs_ooc_blockbidiag": [i32, i64, i32, vp, i64, i32, ctypes.POINTER(Opts)],
             "avs_ooc_tiled_qr": [i32, i64, i32, vp, i64, i32, ctypes.POINTER(Opts)],
+            "avs_apply_q": [vp, vp, i64, i64, i32],
             "avs_gen_gaussian_host": [ctypes.c_uint64, i64, ctypes.c_double, i32, vp, i64],
             "avs_debug_mfma_layout": [i32, dp, ctypes.POINTER(ctypes.c_float)],
             "avs_schedule_stats": [i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(i64), dp],

@@ -6,6 +6,7 @@
     python -m avsvd.suites batched   [--count 4096] [--plain-sample 128]
     python -m avsvd.suites serving   [--requests 48] [--budget-gb 64]
     python -m avsvd.suites ooc       [--n 16384] [--b 128] [--kswitches 32 64]
+    python -m avsvd.suites qr        [--ns 1024 4096 16384] [--bs 64 128] [--dtypes f32 f64]
 
 Every case prints one JSON line as soon as it finishes.  Inputs are synthetic
 and seeded (Philox4x32-10 + Box-Muller from the library, seed 20261203 + case
@@ -442,6 +443,136 @@ def suite_ooc(args):
         emit(d)
 
 
+class RocSolver:
+    """The vendor QR: rocSOLVER ?geqrf / ?ormqr through the copies torch ships
+    (built on the HIP runtime torch and this library share)."""
+
+    def __init__(self, stream):
+        import torch
+        d = os.path.join(os.path.dirname(torch.__file__), "lib")
+        self.blas = ctypes.CDLL(os.path.join(d, "librocblas.so"))
+        self.sol = ctypes.CDLL(os.path.join(d, "librocsolver.so"))
+        self.h = ctypes.c_void_p()
+        if self.blas.rocblas_create_handle(ctypes.byref(self.h)) != 0:
+            raise RuntimeError("rocblas_create_handle failed")
+        if self.blas.rocblas_set_stream(self.h, ctypes.c_void_p(stream)) != 0:
+            raise RuntimeError("rocblas_set_stream failed")
+        vp, i = ctypes.c_void_p, ctypes.c_int
+        for name in ("rocsolver_dgeqrf", "rocsolver_sgeqrf"):
+            getattr(self.sol, name).argtypes = [vp, i, i, vp, i, vp]
+        for name in ("rocsolver_dormqr", "rocsolver_sormqr"):
+            getattr(self.sol, name).argtypes = [vp, i, i, i, i, i, vp, i, vp, vp, i]
+
+    def geqrf(self, dt, n, pA, ptau):
+        f = self.sol.rocsolver_dgeqrf if dt == "f64" else self.sol.rocsolver_sgeqrf
+        if f(self.h, n, n, pA, n, ptau) != 0:
+            raise RuntimeError("rocsolver geqrf failed")
+
+    def ormqr_qt(self, dt, n, m, pA, ptau, pC):
+        # side left (141), transpose (112): C <- Q^T C
+        f = self.sol.rocsolver_dormqr if dt == "f64" else self.sol.rocsolver_sormqr
+        if f(self.h, 141, 112, n, m, n, pA, n, ptau, pC, n) != 0:
+            raise RuntimeError("rocsolver ormqr failed")
+
+    def close(self):
+        self.blas.rocblas_destroy_handle(self.h)
+
+
+def _timed(fn, stream, reps):
+    """median milliseconds of fn() on `stream` over reps runs (after one warmup)."""
+    import torch
+    fn()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def suite_qr(args):
+    """tiled_QR! and myormqr! against the vendor geqrf / ormqr, as
+    CAQR/src/benchmark.jl:14-47 compares with cuSOLVER: |R| and |Q^T C|
+    differences relative to the vendor's, median times, vendor time / ours."""
+    import torch
+    # one stream for torch's copies, rocSOLVER and the handle (a handle given
+    # stream 0 would make a stream of its own, unordered with the others)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        _qr_cases(args, stream)
+
+
+def _qr_cases(args, stream):
+    import torch
+    rs = RocSolver(stream.cuda_stream)
+    for dt in args.dtypes:
+        tdt = torch.float64 if dt == "f64" else torch.float32
+        for n in args.ns:
+            g = torch.Generator(device="cuda:0").manual_seed(SEED + n)
+            A0 = torch.randn(n, n, device="cuda:0", dtype=tdt, generator=g)  # column-major n x n in memory
+            C0 = torch.randn(n, n, device="cuda:0", dtype=tdt, generator=g)
+            tau = torch.zeros(n, device="cuda:0", dtype=tdt)
+            Av, Cv = A0.clone(), C0.clone()
+            t_geqrf = _timed(lambda: (Av.copy_(A0), rs.geqrf(dt, n, Av.data_ptr(), tau.data_ptr())), stream, args.reps)
+            Av.copy_(A0)
+            rs.geqrf(dt, n, Av.data_ptr(), tau.data_ptr())
+            t_ormqr = _timed(lambda: (Cv.copy_(C0), rs.ormqr_qt(dt, n, n, Av.data_ptr(), tau.data_ptr(), Cv.data_ptr())),
+                             stream, args.reps)
+            Cv.copy_(C0)
+            rs.ormqr_qt(dt, n, n, Av.data_ptr(), tau.data_ptr(), Cv.data_ptr())
+            torch.cuda.synchronize()
+            Rv = torch.triu(Av.T).abs()   # the memory is column-major: rows of Av.T are matrix rows
+            QtCv = Cv.abs()
+            t_copy = _timed(lambda: Av.copy_(A0), stream, args.reps)
+            for b in args.bs:
+                o = Opts()
+                lib().avs_default_opts(ctypes.byref(o))
+                o.stream = stream.cuda_stream
+                h = ctypes.c_void_p()
+                check(lib().avs_create(ctypes.byref(h), AVS_F64 if dt == "f64" else AVS_F32, n, b, ctypes.byref(o)))
+                A = A0.clone()
+                C = C0.clone()
+
+                def ours_qr():
+                    check(lib().avs_set_matrix_device(h, ctypes.c_void_p(A0.data_ptr()), n))
+                    check(lib().avs_tiled_qr(h, 0))
+                    check(lib().avs_get_matrix_device(h, ctypes.c_void_p(A.data_ptr()), n))
+
+                def ours_q():
+                    C.copy_(C0)
+                    check(lib().avs_apply_q(h, ctypes.c_void_p(C.data_ptr()), n, n, 1))
+
+                t_qr = _timed(ours_qr, stream, args.reps)
+                kern = ctypes.c_double()
+                check(lib().avs_last_time_ms(h, ctypes.byref(kern)))
+                t_q = _timed(ours_q, stream, args.reps)
+                torch.cuda.synchronize()
+                R = torch.triu(A.T).abs()
+                err_r = float(torch.linalg.norm((R - Rv).double()) / torch.linalg.norm(Rv.double()))
+                err_q = float(torch.linalg.norm((C.abs() - QtCv).double()) / torch.linalg.norm(QtCv.double()))
+                lib().avs_destroy(h)
+                fl = 4.0 / 3.0 * n ** 3
+                emit({"suite": "qr", "dtype": dt, "n": n, "b": b,
+                      "geqrf_ms": round(t_qr, 3), "geqrf_kernels_ms": round(kern.value, 3),
+                      "vendor_geqrf_ms": round(t_geqrf - t_copy, 3),
+                      "vendor_over_ours_geqrf": round((t_geqrf - t_copy) / t_qr, 3),
+                      "geqrf_tflops": round(fl / t_qr / 1e9, 3),
+                      "rel_err_abs_R_vs_vendor": err_r,
+                      "ormqr_ms": round(t_q - t_copy, 3), "vendor_ormqr_ms": round(t_ormqr - t_copy, 3),
+                      "vendor_over_ours_ormqr": round((t_ormqr - t_copy) / max(t_q - t_copy, 1e-9), 3),
+                      "rel_err_abs_QtC_vs_vendor": err_q,
+                      "vendor": "rocSOLVER ?geqrf / ?ormqr (torch's copy), same stream",
+                      "note": "our geqrf time includes packing A into tiles and writing R and the reflectors "
+                              "back to an n x n array; the vendor's and both ormqr times are medians minus the "
+                              "n x n device copy that restores their input"})
+            del A0, C0, Av, Cv
+            torch.cuda.empty_cache()
+    rs.close()
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="suite", required=True)
@@ -468,13 +599,18 @@ def main(argv=None):
     p.add_argument("--inflight", type=int, default=4)
     p.add_argument("--device-sigma", action="store_true",
                    help="bidiagonals' singular values by bisection on the device in the serving loop")
+    p = sub.add_parser("qr")
+    p.add_argument("--ns", type=int, nargs="+", default=[1024, 2048, 4096, 8192, 16384])
+    p.add_argument("--bs", type=int, nargs="+", default=[64, 128])
+    p.add_argument("--dtypes", nargs="+", default=["f32", "f64"])
+    p.add_argument("--reps", type=int, default=5)
     p = sub.add_parser("ooc")
     p.add_argument("--n", type=int, default=16384)
     p.add_argument("--b", type=int, default=128)
     p.add_argument("--kswitches", type=int, nargs="+", default=[32, 64])
     args = ap.parse_args(argv)
     {"plain": suite_plain, "stage2": suite_stage2, "precision": suite_precision, "batched": suite_batched,
-     "serving": suite_serving, "ooc": suite_ooc}[args.suite](args)
+     "serving": suite_serving, "ooc": suite_ooc, "qr": suite_qr}[args.suite](args)
 
 
 if __name__ == "__main__":

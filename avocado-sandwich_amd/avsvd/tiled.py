@@ -110,6 +110,14 @@ class TiledMatrix:
         return out
 
 
+def apply_q_(A, dC, ldc, ncols, applyt=True):
+    """myormqr!(C, A, Tau) (CAQR/src/qr_kernels.jl:256-265): C <- Q^T C (or Q C)
+    on the device matrix at address dC (column-major, leading dimension ldc,
+    ncols columns, A's dtype), Q from A's last tiled_QR_; returns when done."""
+    check(lib().avs_apply_q(A._h, ctypes.c_void_p(dC), int(ldc), int(ncols), 1 if applyt else 0))
+    A.synchronize()
+
+
 def _kend(A, kend):
     return 0 if kend is None else int(kend)
 

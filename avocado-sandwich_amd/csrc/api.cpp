@@ -840,6 +840,49 @@ avs_status avs_get_matrix_host(avs_handle *h, void *A, int64_t lda)
 avs_status avs_blockbidiag(avs_handle *h, int32_t kend) { return run_alg(h, ALG_BLOCKBIDIAG, kend); }
 avs_status avs_tiled_qr(avs_handle *h, int32_t kend) { return run_alg(h, ALG_TILED_QR, kend); }
 
+// myormqr! (CAQR/src/qr_kernels.jl:256-265): the reflectors of every panel
+// operation of the handle's last tiled_QR!, in program order (Q^T) or reverse
+// (Q), applied to the tile rows of C they act on, over every column tile of C.
+// The phase program is rebuilt with the run's tree, domain and kend: its
+// factor slots are the ones the run filled.
+avs_status avs_apply_q(avs_handle *h, void *dC, int64_t ldc, int64_t ncols, int32_t applyt)
+{
+    if (!h) return fail(-1, "null handle");
+    if (!dC) return fail(-2, "null C");
+    if (ldc < h->n) return fail(-3, "ldc < n");
+    if (ncols < 0 || ncols % h->b != 0) return fail(-4, "ncols must be a non-negative multiple of b");
+    if (h->state != avs_handle::ST_QR)
+        return fail(AVS_ERR_STATE, "no tiled_QR! factors on the handle (run avs_tiled_qr first)");
+    if (ncols == 0) return AVS_OK;
+    PhaseProgram P;
+    build_phase_program(P, h->nt, h->b, ALG_TILED_QR, h->band_k, resolve_tree(h), resolve_domain(h));
+    std::vector<ChainOp> pan;
+    for (const Phase &ph : P.phases) {
+        if (ph.kind != TK_PANEL) continue;
+        for (int c = ph.chain0; c < ph.chain0 + ph.nchains; ++c)
+            for (int o = P.chain_start[c]; o < P.chain_start[c + 1]; ++o) pan.push_back(P.ops[o]);
+    }
+    if (!applyt) std::reverse(pan.begin(), pan.end());
+    const int ntc = (int)(ncols / h->b);
+    return dispatch_tb(h->dtype, h->b, [&](auto tp, auto bc) -> avs_status {
+        typedef std::remove_pointer_t<decltype(tp)> TS;
+        constexpr int B = decltype(bc)::value;
+        const long tsz = (long)B * B, cstep = (long)B * ldc;
+        TS *tiles = (TS *)h->dTiles, *C = (TS *)dC;
+        for (const ChainOp &op : pan) {
+            // GEQRT reflectors sit in the tile they factor; TSQRT and merge
+            // reflectors in the bottom tile.  Tile id = column * nt + row.
+            const TS *V = tiles + (long)(op.mode == MODE_GE ? op.vtile : op.btile) * tsz;
+            TS *C1 = C + (long)(op.vtile % h->nt) * B;
+            TS *C2 = C + (long)(op.btile % h->nt) * B;
+            launch_update<TS, B>(op.mode, applyt != 0, V, B, 0, h->dTjj + (long)op.fac * 16 * B, C1, ldc, cstep, C2,
+                                 ldc, cstep, 0, ntc, upd_tiles_for(B), split_of(h), h->stream);
+        }
+        HIPCHK(hipGetLastError());
+        return AVS_OK;
+    });
+}
+
 avs_status avs_synchronize(avs_handle *h)
 {
     if (!h) return fail(-1, "null handle");

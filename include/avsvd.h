@@ -129,6 +129,13 @@ avs_status avs_get_matrix_device(avs_handle *h, void *dA, int64_t lda);
 avs_status avs_blockbidiag(avs_handle *h, int32_t kend);
 /* tiled_QR!(A; kend). */
 avs_status avs_tiled_qr(avs_handle *h, int32_t kend);
+/* myormqr!(C, A, Tau) (CAQR/src/qr_kernels.jl:256-265): C <- Q^T C (applyt != 0) or
+ * Q C, with Q the orthogonal factor of the handle's last avs_tiled_qr (its reflectors and
+ * compact-WY factors stay in the handle until the next run), on the n x ncols column-major
+ * device matrix dC (leading dimension ldc >= n, ncols a multiple of b, the handle's dtype),
+ * enqueued on the handle's stream.  Errors: -2 null dC, -3 ldc < n, -4 ncols,
+ * AVS_ERR_STATE when the handle holds no tiled_QR! factors. */
+avs_status avs_apply_q(avs_handle *h, void *dC, int64_t ldc, int64_t ncols, int32_t applyt);
 
 /* ---- out-of-core runs on a host matrix (OOC_alg!, KAbasedSVD/src/tiledalgos.jl:232-251) ----
  * avs_ooc_blockbidiag is OOC_Bidiag! (:254), avs_ooc_tiled_qr is OOC_QR! (:253): the

@@ -16,7 +16,7 @@
 # ErrorException carrying avs_last_error().
 module AvSVD
 
-export TiledMatrix, tiled_QR!, Blockbidiag!, OOC_QR!, OOC_Bidiag!, OOC_SVD!, compile_kernels,
+export TiledMatrix, tiled_QR!, myormqr!, Blockbidiag!, OOC_QR!, OOC_Bidiag!, OOC_SVD!, compile_kernels,
        bidiagonalize, my_CU_svdval, diag_lapack, svdvals_gpu, tune_set!
 
 const libavsvd = get(ENV, "AVSVD_LIB", joinpath(@__DIR__, "..", "libavsvd.so"))
@@ -130,6 +130,16 @@ end
 function tiled_QR!(A::TiledMatrix; kend=A.no_tiles)
     check(ccall((:avs_tiled_qr, libavsvd), Int32, (Ptr{Cvoid}, Int32), A.handle, kend))
     return sync_to_host!(A)
+end
+
+# myormqr!(C, A)  (CAQR/src/qr_kernels.jl:256-265): C <- Q' C (applyt) or Q C for the Q of
+# A's last tiled_QR!, on a device array C (column-major, n x ncols, ncols a multiple of b)
+function myormqr!(C::AbstractMatrix{T}, A::TiledMatrix{T}; applyt::Bool=true) where {T}
+    C isa Array && error("myormqr!: C must be a device array")
+    check(ccall((:avs_apply_q, libavsvd), Int32, (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Int32),
+                A.handle, Ptr{Cvoid}(pointer(C)), stride(C, 2), size(C, 2), Int32(applyt)))
+    check(ccall((:avs_synchronize, libavsvd), Int32, (Ptr{Cvoid},), A.handle))
+    return C
 end
 
 # Blockbidiag!(A; kend)  (tiledalgos.jl:17-34)

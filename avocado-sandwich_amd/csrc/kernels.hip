@@ -135,8 +135,9 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
     const int t = threadIdx.x;
     const long tsz = (long)B * B;
     for (;;) {
+        // a failed wait anywhere ends the launch: no new ticket once status is set
         if (t == 0) {
-            sh_task = atomicAdd(ticket, 1);
+            sh_task = ld_relaxed(status) != 0 ? ntasks : atomicAdd(ticket, 1);
             sh_fail = 0;
         }
         __syncthreads();
@@ -154,7 +155,10 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
                 const int need = dep_val[T.dep0 + d];
                 while (ld_relaxed(p) < need) {
                     __builtin_amdgcn_s_sleep(1);
-                    if (wall_clock64() - t0 > wait_ticks) {
+                    // the bound covers the whole launch, not each wait: a
+                    // workgroup whose wait ran out sets status, and every
+                    // other wait gives up as soon as it sees that
+                    if (wall_clock64() - t0 > wait_ticks || ld_relaxed(status) != 0) {
                         sh_fail = 1;
                         break;
                     }
@@ -165,7 +169,8 @@ __global__ void __launch_bounds__(NTHREADS, 1) persistent_kernel(TS *tiles, doub
         }
         __syncthreads();
         if (sh_fail) {
-            if (t == 0) atomicExch(status, 1 + tk);
+            // the first failure keeps its task index
+            if (t == 0) atomicCAS(status, 0, 1 + tk);
             break;
         }
         double *tau = taus + (long)T.fac * B;

@@ -7,6 +7,7 @@
     python -m avsvd.suites serving   [--requests 48] [--budget-gb 64]
     python -m avsvd.suites ooc       [--n 16384] [--b 128] [--kswitches 32 64]
     python -m avsvd.suites qr        [--ns 1024 4096 16384] [--bs 64 128] [--dtypes f32 f64]
+    python -m avsvd.suites phases    [--n 16384] [--dtype f32] [--bs 64 128] [--modes native split]
 
 Every case prints one JSON line as soon as it finishes.  Inputs are synthetic
 and seeded (Philox4x32-10 + Box-Muller from the library, seed 20261203 + case
@@ -573,6 +574,42 @@ def _qr_cases(args, stream):
     rs.close()
 
 
+def suite_phases(args):
+    """Step time with the summed panel and update kernel times (per-launch
+    events) on the stream-ordered executor, per tile size and update form."""
+    import torch
+    n = args.n
+    dt = np.float64 if args.dtype == "f64" else np.float32
+    A = gaussian_matrix(n, SEED, dtype=dt)
+    dA = torch.from_numpy(A.T).to("cuda:0")
+    torch.cuda.synchronize()
+    for b in args.bs:
+        for mode in args.modes:
+            o = Opts()
+            lib().avs_default_opts(ctypes.byref(o))
+            o.sched, o.tree = SCHED_STREAMS, TREE_HIER
+            o.update_mode = UPD_SPLIT_BF16 if mode == "split" else UPD_NATIVE
+            o.phase_timing, o.reserve_cus, o.masked_modes = 1, args.reserve_cus, args.masked_modes
+            h = ctypes.c_void_p()
+            check(lib().avs_create(ctypes.byref(h), AVS_F64 if args.dtype == "f64" else AVS_F32, n, b, ctypes.byref(o)))
+            for s in range(args.runs):
+                check(lib().avs_set_matrix_device(h, ctypes.c_void_p(dA.data_ptr()), n))
+                check(lib().avs_blockbidiag(h, 0))
+                ms = ctypes.c_double()
+                check(lib().avs_last_time_ms(h, ctypes.byref(ms)))
+                pm, um, uf = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+                pn, un = ctypes.c_int64(), ctypes.c_int64()
+                check(lib().avs_phase_timing(h, ctypes.byref(pm), ctypes.byref(pn), ctypes.byref(um), ctypes.byref(un),
+                                             ctypes.byref(uf)))
+                if s == 0:
+                    continue  # the first run builds the schedule
+                emit({"suite": "phases", "n": n, "dtype": args.dtype, "b": b, "updates": mode, "run": s,
+                      "step_ms": round(ms.value, 2), "panel_kernels_ms": round(pm.value, 2),
+                      "panel_launches": pn.value, "update_kernels_ms": round(um.value, 2),
+                      "update_launches": un.value, "update_tflops": round(uf.value / (um.value * 1e-3) / 1e12, 2)})
+            lib().avs_destroy(h)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="suite", required=True)
@@ -599,6 +636,14 @@ def main(argv=None):
     p.add_argument("--inflight", type=int, default=4)
     p.add_argument("--device-sigma", action="store_true",
                    help="bidiagonals' singular values by bisection on the device in the serving loop")
+    p = sub.add_parser("phases")
+    p.add_argument("--n", type=int, default=16384)
+    p.add_argument("--dtype", choices=["f64", "f32"], default="f32")
+    p.add_argument("--bs", type=int, nargs="+", default=[64, 128])
+    p.add_argument("--modes", nargs="+", default=["native", "split"])
+    p.add_argument("--runs", type=int, default=3)
+    p.add_argument("--reserve-cus", type=int, default=16)
+    p.add_argument("--masked-modes", type=int, default=2)
     p = sub.add_parser("qr")
     p.add_argument("--ns", type=int, nargs="+", default=[1024, 2048, 4096, 8192, 16384])
     p.add_argument("--bs", type=int, nargs="+", default=[64, 128])
@@ -610,7 +655,8 @@ def main(argv=None):
     p.add_argument("--kswitches", type=int, nargs="+", default=[32, 64])
     args = ap.parse_args(argv)
     {"plain": suite_plain, "stage2": suite_stage2, "precision": suite_precision, "batched": suite_batched,
-     "serving": suite_serving, "ooc": suite_ooc, "qr": suite_qr}[args.suite](args)
+     "serving": suite_serving, "ooc": suite_ooc, "qr": suite_qr,
+     "phases": suite_phases}[args.suite](args)
 
 
 if __name__ == "__main__":

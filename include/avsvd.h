@@ -176,8 +176,22 @@ avs_status avs_phase_trace(avs_handle *h, double *start_ms, double *end_ms, int3
  * ldband >= b+1.  Only (b+1) x n values cross PCIe, not n x n. */
 avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband);
 
-/* Full svdvals: band reduction (if not already run), band to host, block
- * bulge chasing and dbdsdc in FP64 on the host.  sigma: n values, descending. */
+/* Full svdvals (OOC_SVD! + diag_lapack, KAbasedSVD/src/tiledalgos.jl:256-260,
+ * bulgechasing.jl:50-53): sigma[n], descending, FP64.
+ *  1. Band reduction of the handle's matrix if it has not run to the end (after a
+ *     partial avs_blockbidiag the remaining steps run on the band-zeroed matrix, as
+ *     Blockbidiag! on the reference's array would).
+ *  2. Band -> bidiagonal in FP64 by Householder bulge chasing: on the device
+ *     (avs_bidiagonalize_band_device, no download of the band) for
+ *     n >= AVSVD_DEVICE_CHASE_MIN_N (default 16384) and 2 <= b <= 128, otherwise on
+ *     the host after avs_get_band (pipelined over the library's host threads).
+ *     AVSVD_STAGE2=device | host force either side; AVSVD_STAGE2=halving runs the
+ *     reference's block sweeps (bulgechasing.jl:33-42) and =dgbbrd LAPACK's chase.
+ *  3. Singular values of the bidiagonal: Sturm-count bisection on the device for
+ *     n >= 128, LAPACK dbdsdc('U','N') on the host below that or with AVSVD_BDSDC=host.
+ * Errors: AVS_ERR_STATE if no matrix is set or the handle holds tiled_QR! factors
+ * (set the matrix again first); AVS_ERR_LAPACK if the host LAPACK is needed and
+ * missing. */
 avs_status avs_svdvals(avs_handle *h, double *sigma);
 
 avs_status avs_synchronize(avs_handle *h);

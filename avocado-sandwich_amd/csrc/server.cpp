@@ -21,6 +21,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -53,7 +55,8 @@ struct Request {
     size_t footprint = 0;
     std::vector<char> A;
     std::vector<double> band, sigma, d, e;
-    double t_arrive = 0, t_admit = 0, t_band = 0, t_done = 0;
+    double t_arrive = 0, t_admit = 0, t_started = 0, t_band = 0, t_done = 0;
+    bool new_handle = false;
     avs_status status = AVS_OK;
     std::string err;
     bool done = false;
@@ -92,6 +95,17 @@ int pick_b(const avs_server_opts &o, int dtype, int64_t n)
     return tuned_b(dtype, n);
 }
 
+// The order a request is padded to (zero rows and columns, which add exact
+// zero singular values that are dropped): 8 sizes per octave, multiples of b,
+// so requests of nearby orders share pooled handles.  At most 12.5 % padding.
+int64_t bucket_of(int64_t n, int b)
+{
+    int64_t p = 1;
+    while (p * 2 <= n) p *= 2;
+    const int64_t step = std::max<int64_t>(b, (p / 8 + b - 1) / b * b);
+    return (n + step - 1) / step * step;
+}
+
 size_t footprint_of(int dtype, int64_t N, int b)
 {
     const size_t es = dtype == AVS_F64 ? 8 : 4;
@@ -106,6 +120,7 @@ size_t footprint_of(int dtype, int64_t N, int b)
 struct Slot {
     avs_handle *h = nullptr;
     hipEvent_t ev = nullptr;
+    void *hpin = nullptr; // pinned host copy of the request's matrix (n x n, ld n), so the upload is a DMA
     int dtype = AVS_F64;
     int64_t N = 0;
     int b = 0;
@@ -195,8 +210,20 @@ void destroy_slot(Slot &x)
 {
     if (x.h) avs_destroy(x.h);
     if (x.ev) hipEventDestroy(x.ev);
+    if (x.hpin) hipHostFree(x.hpin);
     x.h = nullptr;
     x.ev = nullptr;
+    x.hpin = nullptr;
+}
+
+// AVSVD_SERVER_TRACE=1: one line per finished request on stderr
+bool trace_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("AVSVD_SERVER_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
 }
 
 // marks a request done (with its status) and wakes its waiter
@@ -205,6 +232,13 @@ void finish(avs_server *S, const std::shared_ptr<Request> &r)
     {
         std::lock_guard<std::mutex> lk(S->mu);
         r->t_done = now_ms();
+        if (trace_on())
+            fprintf(stderr,
+                    "avs_server: request %llu n %lld N %lld %s b %d: queued %.1f ms, set-up %.1f ms%s, device %.1f ms, "
+                    "host stage %.1f ms, status %d\n",
+                    (unsigned long long)r->id, (long long)r->n, (long long)r->N, r->dtype == AVS_F64 ? "f64" : "f32",
+                    r->b, r->t_admit - r->t_arrive, r->t_started - r->t_admit, r->new_handle ? " (new handle)" : "",
+                    r->t_band - r->t_started, r->t_done - r->t_band, (int)r->status);
         r->done = true;
         if (!r->status) {
             S->lat.push_back(r->t_done - r->t_arrive);
@@ -230,6 +264,7 @@ avs_status start(avs_server *S, Slot &x, Running &run)
 {
     const Request &r = *run.r;
     avs_status st = AVS_OK;
+    run.r->new_handle = !x.h;
     if (!x.h) {
         avs_opts o;
         avs_default_opts(&o);
@@ -237,12 +272,18 @@ avs_status start(avs_server *S, Slot &x, Running &run)
         if ((st = avs_create(&x.h, r.dtype, r.N, r.b, &o))) return st;
         if (hipEventCreateWithFlags(&x.ev, hipEventDisableTiming) != hipSuccess)
             return fail(AVS_ERR_HIP, "server: event creation failed");
+        const size_t es = r.dtype == AVS_F64 ? 8 : 4;
+        if (hipHostMalloc(&x.hpin, es * (size_t)r.N * r.N, hipHostMallocDefault) != hipSuccess)
+            return fail(AVS_ERR_ALLOC, "server: pinned staging of %zu bytes", es * (size_t)r.N * r.N);
     }
-    if ((st = avs_set_matrix_host_sub(x.h, r.A.data(), r.n, r.n))) return st;
+    // the slot is idle, so its previous upload from hpin has completed
+    memcpy(x.hpin, r.A.data(), r.A.size());
+    if ((st = avs_set_matrix_host_sub(x.h, x.hpin, r.n, r.n))) return st;
     if ((st = avs_blockbidiag(x.h, 0))) return st;
     if ((st = avs::band_enqueue(x.h, &run.band))) return st;
     if (hipEventRecord(x.ev, (hipStream_t)avs::handle_stream(x.h)) != hipSuccess)
         return fail(AVS_ERR_HIP, "server: event record failed");
+    run.r->t_started = now_ms();
     return AVS_OK;
 }
 
@@ -426,7 +467,9 @@ void host_worker(avs_server *S)
         r->d.assign(N, 0.0);
         r->e.assign(std::max<int64_t>(N - 1, 1), 0.0);
         // one host thread per request, four from n = 4096 (the largest
-        // requests set the p99; the sweeps pipeline well there)
+        // requests set the p99; the sweeps pipeline well there).  Every host
+        // thread from n = 8192 was measured slower at saturation: the
+        // overlapping large requests then oversubscribe the cores.
         avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, r->d.data(), r->e.data(),
                                                     N >= 4096 ? 4 : 1);
         std::vector<double>().swap(r->band);
@@ -533,7 +576,7 @@ avs_status avs_server_submit(avs_server *S, int32_t dtype, int64_t n, const void
     r->dtype = dtype;
     r->n = n;
     r->b = pick_b(S->opts, dtype, n);
-    r->N = (n + r->b - 1) / r->b * r->b;
+    r->N = bucket_of(n, r->b);
     r->footprint = footprint_of(dtype, r->N, r->b);
     if (r->footprint > (size_t)S->opts.hbm_budget_bytes)
         return fail(AVS_ERR_BUDGET, "request n=%lld needs %zu bytes, budget %lld", (long long)n, r->footprint,

@@ -279,7 +279,8 @@ def main():
     if args.serving_requests:
         log("served form: %d requests" % args.serving_requests)
         try:
-            serving = suites.serving_run(args.serving_requests, args.serving_nmax, budget_gb=64, inflight=4)
+            serving = suites.serving_run(args.serving_requests, args.serving_nmax, budget_gb=64, inflight=4,
+                                         device_sigma=True)
         except Exception as e:
             serving = {"error": repr(e)}
         torch.cuda.empty_cache()

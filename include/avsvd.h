@@ -328,10 +328,19 @@ avs_status avs_batch_svdvals(avs_batch *batch, double *sigma);
 /* ---- serving: continuous-batching svdvals request loop ----
  * Requests of any n (FP32 or FP64) are accepted as they arrive and admitted
  * to the device first-come first-served while the HBM budget has room for
- * their working set; up to max_inflight band reductions run concurrently on
- * separate streams.  Only the band crosses PCIe; the host stage runs on
- * host_threads workers.  Latency = submit -> singular values ready.  No
- * reference counterpart (the reference has no serving path). */
+ * their working set; up to max_inflight band reductions run concurrently, each
+ * on the stream of its handle.  A request is zero-padded to its size bucket
+ * (multiples of b, eight sizes per octave; the padding adds zero singular
+ * values, which are dropped), and runs on a pooled handle of that (dtype,
+ * bucket, b): handles, their device buffers and a pinned host staging buffer
+ * are created on first use, kept while the budget allows (least recently used
+ * ones are released for new shapes) and freed by avs_server_destroy.  One
+ * device thread makes every HIP call of the loop; the host_threads workers do
+ * the CPU work (band -> bidiagonal, and dbdsdc without device_sigma).  Only
+ * the band crosses PCIe back.  Latency = submit -> singular values ready.
+ * AVSVD_SERVER_TRACE=1 prints one line per finished request (queueing,
+ * set-up, device and host-stage times) on stderr.  No reference counterpart
+ * (the reference has no serving path). */
 typedef struct avs_server avs_server;
 typedef struct avs_server_opts {
     int64_t hbm_budget_bytes; /* device working-set budget for admitted requests */
@@ -340,8 +349,8 @@ typedef struct avs_server_opts {
     int32_t b_f32, b_f64;     /* tile sizes (0 = the tile-size table below) */
     int32_t update_mode;      /* avs_update_mode for FP32 requests */
     int32_t device_sigma;     /* != 0: the bidiagonal's singular values by bisection on the
-                                 device (each host worker has its own stream and buffers,
-                                 allocated at server creation); 0: LAPACK dbdsdc on the
+                                 device, run by the device thread on max_inflight streams with
+                                 buffers allocated at server creation; 0: LAPACK dbdsdc on the
                                  host worker */
     int32_t reserved[6];      /* must be 0 */
 } avs_server_opts;

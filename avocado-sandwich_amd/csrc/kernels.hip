@@ -2,8 +2,12 @@
 // device.hpp: standalone operator kernels (the host-enqueued plain path and the
 // tile-operator ABI), layout conversion, band extraction and the persistent
 // dependency-counting executor of the tile DAG.
+#include <stdlib.h>
+#include <string.h>
+
 #include "device.hpp"
 #include "kernels.hpp"
+#include "panel_blk.hpp"
 
 namespace avs {
 
@@ -86,6 +90,26 @@ __global__ void __launch_bounds__(PanelThreads<B>::value) panel_kernel(TS *top, 
 {
     __shared__ double smem[Lds<TS, B>::panel];
     panel_body<TS, B, MODE, PanelThreads<B>::value>(top, ldt, bot, ldb, tr != 0, tau, tjj, smem);
+}
+
+// the blocked panel of 128 x 128 tiles (panel_blk.hpp)
+template <typename TS, int MODE>
+__global__ void __launch_bounds__(PBLK_THREADS) panel_blk_kernel(TS *top, long ldt, TS *bot, long ldb, int tr,
+                                                                 double *tau, double *tjj)
+{
+    __shared__ double smem[LdsBlk<128>::value];
+    panel_blk_body<TS, MODE>(top, ldt, bot, ldb, tr != 0, tau, tjj, smem);
+}
+
+// b = 128 panels use the blocked body unless AVSVD_PANEL=rows selects the
+// one-barrier-per-reflector body (read once per process)
+static bool panel_blk_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("AVSVD_PANEL");
+        return !(e && strcmp(e, "rows") == 0);
+    }();
+    return on;
 }
 
 template <typename TS, int B, int MODE>
@@ -260,10 +284,42 @@ __global__ void __launch_bounds__(UPD_THREADS, 2) chain_update_kernel(TS *tiles,
     }
 }
 
+template <typename TS, int MODE>
+__global__ void __launch_bounds__(PBLK_THREADS) chain_panel_blk_kernel(TS *tiles, double *taus, double *tjjs,
+                                                                       const ChainOp *__restrict__ ops,
+                                                                       const int *__restrict__ chain_start, int chain0)
+{
+    constexpr int B = 128;
+    __shared__ double smem[LdsBlk<B>::value];
+    const long tsz = (long)B * B;
+    const int c = chain0 + blockIdx.x;
+    for (int o = chain_start[c]; o < chain_start[c + 1]; ++o) {
+        const ChainOp T = ops[o];
+        TS *top = tiles + (long)T.vtile * tsz;
+        TS *bot = tiles + (long)T.btile * tsz;
+        double *tau = taus + (long)T.fac * B;
+        double *tjj = tjjs + (long)T.fac * 16 * B;
+        panel_blk_body<TS, MODE>(top, B, MODE == MODE_GE ? top : bot, B, T.trans != 0, tau, tjj, smem);
+        __syncthreads();
+    }
+}
+
 template <typename TS, int B>
 void launch_chain_panel(int nchains, int mode, void *tiles, double *taus, double *tjjs, const ChainOp *ops,
                         const int *chain_start, int chain0, hipStream_t s)
 {
+    if constexpr (B == 128) {
+        if (panel_blk_on()) {
+#define AVS_CHB(M)                                                                                                     \
+    hipLaunchKernelGGL((chain_panel_blk_kernel<TS, M>), dim3(nchains), dim3(PBLK_THREADS), 0, s, (TS *)tiles, taus,    \
+                       tjjs, ops, chain_start, chain0)
+            if (mode == MODE_GE) AVS_CHB(MODE_GE);
+            else if (mode == MODE_TS) AVS_CHB(MODE_TS);
+            else AVS_CHB(MODE_TT);
+#undef AVS_CHB
+            return;
+        }
+    }
 #define AVS_CHP(M)                                                                                                     \
     hipLaunchKernelGGL((chain_panel_kernel<TS, B, M>), dim3(nchains), dim3(PanelThreads<B>::value), 0, s, (TS *)tiles, \
                        taus, tjjs, ops, chain_start, chain0)
@@ -520,6 +576,20 @@ template <typename TS> void launch_band(const void *tiles, double *band, long ld
 template <typename TS, int B>
 void launch_panel(int mode, void *top, long ldt, void *bot, long ldb, int tr, double *tau, double *tjj, hipStream_t s)
 {
+    if constexpr (B == 128) {
+        if (panel_blk_on()) {
+            if (mode == MODE_GE)
+                hipLaunchKernelGGL((panel_blk_kernel<TS, MODE_GE>), dim3(1), dim3(PBLK_THREADS), 0, s, (TS *)top, ldt,
+                                   (TS *)top, ldt, tr, tau, tjj);
+            else if (mode == MODE_TS)
+                hipLaunchKernelGGL((panel_blk_kernel<TS, MODE_TS>), dim3(1), dim3(PBLK_THREADS), 0, s, (TS *)top, ldt,
+                                   (TS *)bot, ldb, tr, tau, tjj);
+            else
+                hipLaunchKernelGGL((panel_blk_kernel<TS, MODE_TT>), dim3(1), dim3(PBLK_THREADS), 0, s, (TS *)top, ldt,
+                                   (TS *)bot, ldb, tr, tau, tjj);
+            return;
+        }
+    }
     if (mode == MODE_GE)
         hipLaunchKernelGGL((panel_kernel<TS, B, MODE_GE>), dim3(1), dim3(PanelThreads<B>::value), 0, s, (TS *)top, ldt, (TS *)top,
                            ldt, tr, tau, tjj);

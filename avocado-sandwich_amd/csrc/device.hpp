@@ -845,11 +845,16 @@ template <typename TS, int B> struct Lds2 {
     static constexpr int upd = (2 * B * YS * (int)sizeof(TS) + 16 * B * (int)sizeof(TS) + 7) / 8;
 };
 
-template <typename TS, int B, bool APPLYT, int NT, bool CTR>
+// SPW = 1: a half-tile workgroup for launches too small to fill the GPU:
+// wave w takes strip s0 + w alone (s0 = 0 or NB/2, ntiles = 1), so one column
+// tile is spread over two workgroups and twice as many SIMDs.
+template <typename TS, int B, bool APPLYT, int NT, bool CTR, int SPW = 2>
 __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V, long ldv, bool vtr,
                                              const double *__restrict__ tjj, TS *__restrict__ C1, long ld1, long s1,
-                                             TS *__restrict__ C2, long ld2, long s2, int ntiles, double *smem)
+                                             TS *__restrict__ C2, long ld2, long s2, int ntiles, double *smem,
+                                             int s0 = 0)
 {
+    static_assert(SPW == 1 || SPW == 2, "update2 strips per wave");
     typedef typename AccT<TS>::type acc_t;
     constexpr int NB = B / 16, YS = Lds2<TS, B>::YS;
     // Rows of C (and reflectors of block J) are taken in "memory order": in
@@ -909,19 +914,20 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
     const long rs2 = CTR ? ld2 : 1, cs2 = CTR ? 1 : ld2;
     const int pil = perm(il);
     const int nstrips = ntiles * NB;
-    const bool mine = 2 * w < nstrips; // wave-uniform; strips come in pairs (NB is even)
-    TS *c1p[2], *c2p[2];
+    // wave-uniform; strips come in pairs (NB is even)
+    const bool mine = SPW == 2 ? 2 * w < nstrips : w < NB / 2;
+    TS *c1p[SPW], *c2p[SPW];
 #pragma unroll
-    for (int sp = 0; sp < 2; ++sp) {
-        const int strip = 2 * w + sp;
+    for (int sp = 0; sp < SPW; ++sp) {
+        const int strip = SPW == 2 ? 2 * w + sp : s0 + w;
         const int tile = strip / NB, col = (strip % NB) * 16 + il;
         c1p[sp] = C1 + (long)tile * s1 + (long)col * cs1 + (long)(4 * q) * rs1;
         c2p[sp] = C2 + (long)tile * s2 + (long)col * cs2 + (long)(4 * q) * rs2;
     }
-    acc_t acc2[2][NB];
+    acc_t acc2[SPW][NB];
     if (mine) {
 #pragma unroll
-        for (int sp = 0; sp < 2; ++sp) {
+        for (int sp = 0; sp < SPW; ++sp) {
 #pragma unroll
             for (int tb = 0; tb < NB; ++tb) {
 #pragma unroll
@@ -940,9 +946,9 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
         if (mine) {
             // C1[J] is requested first and added after the product, so its
             // load overlaps the W MFMAs
-            acc_t c1v[2], Wa[2];
+            acc_t c1v[SPW], Wa[SPW];
 #pragma unroll
-            for (int sp = 0; sp < 2; ++sp) {
+            for (int sp = 0; sp < SPW; ++sp) {
                 Wa[sp] = vzero4<acc_t>();
                 if (mode != MODE_GE) {
 #pragma unroll
@@ -957,28 +963,30 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const TS av = YJ[(16 * tb + 4 * q + m) * YS + pil];
-                    Wa[0] = mfma16(av, acc2[0][tb][m], Wa[0]);
-                    Wa[1] = mfma16(av, acc2[1][tb][m], Wa[1]);
+#pragma unroll
+                    for (int sp = 0; sp < SPW; ++sp) Wa[sp] = mfma16(av, acc2[sp][tb][m], Wa[sp]);
                 }
             }
             if (mode != MODE_GE) {
 #pragma unroll
-                for (int sp = 0; sp < 2; ++sp) {
+                for (int sp = 0; sp < SPW; ++sp) {
 #pragma unroll
                     for (int m = 0; m < 4; ++m) Wa[sp][m] += c1v[sp][m];
                 }
             }
             // W' = T_JJ^T W (Q^T) or T_JJ W (Q)
-            acc_t Wp[2] = {vzero4<acc_t>(), vzero4<acc_t>()};
+            acc_t Wp[SPW];
+#pragma unroll
+            for (int sp = 0; sp < SPW; ++sp) Wp[sp] = vzero4<acc_t>();
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const TS av = APPLYT ? TL[(16 * J + 4 * q + m) * 16 + pil] : TL[(16 * J + pil) * 16 + 4 * q + m];
-                Wp[0] = mfma16(av, Wa[0][m], Wp[0]);
-                Wp[1] = mfma16(av, Wa[1][m], Wp[1]);
+#pragma unroll
+                for (int sp = 0; sp < SPW; ++sp) Wp[sp] = mfma16(av, Wa[sp][m], Wp[sp]);
             }
             if (mode != MODE_GE) {
 #pragma unroll
-                for (int sp = 0; sp < 2; ++sp) {
+                for (int sp = 0; sp < SPW; ++sp) {
 #pragma unroll
                     for (int m = 0; m < 4; ++m) c1p[sp][(long)(16 * J + m) * rs1] = c1v[sp][m] - Wp[sp][m];
                 }
@@ -991,8 +999,8 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const TS av = -YJ[(16 * tb + pil) * YS + 4 * q + m];
-                    acc2[0][tb] = mfma16(av, Wp[0][m], acc2[0][tb]);
-                    acc2[1][tb] = mfma16(av, Wp[1][m], acc2[1][tb]);
+#pragma unroll
+                    for (int sp = 0; sp < SPW; ++sp) acc2[sp][tb] = mfma16(av, Wp[sp][m], acc2[sp][tb]);
                 }
             }
         }
@@ -1001,7 +1009,7 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
     }
     if (mine) {
 #pragma unroll
-        for (int sp = 0; sp < 2; ++sp) {
+        for (int sp = 0; sp < SPW; ++sp) {
 #pragma unroll
             for (int tb = 0; tb < NB; ++tb) {
 #pragma unroll
@@ -1013,19 +1021,21 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
 
 // blocked-staging update with the arithmetic fixed at compile time: split-BF16
 // (FP32 storage only) or the native MFMA of the storage type
-template <typename TS, int B, int MODE, bool APPLYT, bool SPLIT>
+template <typename TS, int B, int MODE, bool APPLYT, bool SPLIT, int SPW = 2>
 __device__ __forceinline__ void update_fixed(const TS *V, long ldv, bool vtr, const double *tjj, TS *C1, long ld1,
-                                             long s1, TS *C2, long ld2, long s2, bool ctr, int ntiles, double *smem)
+                                             long s1, TS *C2, long ld2, long s2, bool ctr, int ntiles, double *smem,
+                                             int s0 = 0)
 {
     if constexpr (SPLIT && sizeof(TS) == 4) {
         update_body_split<B, MODE, APPLYT, UPD_THREADS>((const float *)V, ldv, vtr, tjj, (float *)C1, ld1, s1,
                                                          (float *)C2, ld2, s2, ctr, ntiles, smem);
     } else {
         if (ctr)
-            update2_body<TS, B, APPLYT, UPD_THREADS, true>(MODE, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ntiles, smem);
+            update2_body<TS, B, APPLYT, UPD_THREADS, true, SPW>(MODE, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ntiles,
+                                                                smem, s0);
         else
-            update2_body<TS, B, APPLYT, UPD_THREADS, false>(MODE, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2, ntiles,
-                                                            smem);
+            update2_body<TS, B, APPLYT, UPD_THREADS, false, SPW>(MODE, V, ldv, vtr, tjj, C1, ld1, s1, C2, ld2, s2,
+                                                                 ntiles, smem, s0);
     }
 }
 // the same with the tile mode chosen at run time (one body for all modes) and

@@ -258,7 +258,9 @@ __global__ void __launch_bounds__(PanelThreads<B>::value) chain_panel_kernel(TS 
 // Update phase: workgroup (chain c, column group g) applies every operation of
 // chain c, in order, to the `per` column tiles of group g.  The chain's C1 row
 // tile is rewritten by each operation and stays hot in L2.
-template <typename TS, int B, bool SPLIT, bool TR, int MODE>
+// SPW = 1 (launches too small to fill the GPU, per = 1): two workgroups per
+// column tile, each taking half of its 16-column strips, one strip per wave.
+template <typename TS, int B, bool SPLIT, bool TR, int MODE, int SPW = 2>
 __global__ void __launch_bounds__(UPD_THREADS, 2) chain_update_kernel(TS *tiles, const double *tjjs,
                                                                        const ChainOp *__restrict__ ops,
                                                                        const int *__restrict__ chain_start, int chain0,
@@ -266,8 +268,10 @@ __global__ void __launch_bounds__(UPD_THREADS, 2) chain_update_kernel(TS *tiles,
 {
     __shared__ double smem[LdsU<TS, B, SPLIT>::value];
     const long tsz = (long)B * B;
-    const int c = chain0 + blockIdx.x / groups;
-    const int g = blockIdx.x % groups;
+    constexpr int HALVES = SPW == 1 ? 2 : 1;
+    const int c = chain0 + blockIdx.x / (groups * HALVES);
+    const int g = (blockIdx.x / HALVES) % groups;
+    const int s0 = (blockIdx.x % HALVES) * (B / 32);
     for (int o = chain_start[c]; o < chain_start[c + 1]; ++o) {
         const ChainOp T = ops[o];
         const int first = g * per;
@@ -279,7 +283,7 @@ __global__ void __launch_bounds__(UPD_THREADS, 2) chain_update_kernel(TS *tiles,
         // GE (UNMQR) updates the one tile it is given: C1 = C2
         TS *C1 = MODE == MODE_GE ? C2 : tiles + (long)T.c1tile * tsz + (long)first * st;
         const double *tjj = tjjs + (long)T.fac * 16 * B;
-        update_fixed<TS, B, MODE, true, SPLIT>(V, B, TR, tjj, C1, B, st, C2, B, st, TR, cnt, smem);
+        update_fixed<TS, B, MODE, true, SPLIT, SPW>(V, B, TR, tjj, C1, B, st, C2, B, st, TR, cnt, smem, s0);
         __syncthreads();
     }
 }
@@ -332,6 +336,31 @@ template <typename TS, int B>
 void launch_chain_update(int nchains, int groups, int per, int mode, int trans, void *tiles, const double *tjjs,
                          const ChainOp *ops, const int *chain_start, int chain0, int split, hipStream_t s)
 {
+    // a launch of fewer workgroups than two per CU: half-tile workgroups
+    // (b = 128, one column tile per workgroup, native arithmetic)
+    if constexpr (B == 128) {
+        static const int slots = [] {
+            int dev = 0, ncu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                return 0;
+            return 2 * ncu;
+        }();
+        if (per == 1 && !split && nchains * groups < slots) {
+#define AVS_CHH(TR, M)                                                                                                 \
+    hipLaunchKernelGGL((chain_update_kernel<TS, B, false, TR, M, 1>), dim3(2 * nchains * groups), dim3(UPD_THREADS), 0, \
+                       s, (TS *)tiles, tjjs, ops, chain_start, chain0, groups, per)
+#define AVS_CHH_M(TR)                                                                                                  \
+    if (mode == MODE_GE) AVS_CHH(TR, MODE_GE);                                                                         \
+    else if (mode == MODE_TS) AVS_CHH(TR, MODE_TS);                                                                    \
+    else AVS_CHH(TR, MODE_TT)
+            if (trans) { AVS_CHH_M(true); }
+            else { AVS_CHH_M(false); }
+#undef AVS_CHH_M
+#undef AVS_CHH
+            return;
+        }
+    }
 #define AVS_CHU(SP, TR, M)                                                                                             \
     hipLaunchKernelGGL((chain_update_kernel<TS, B, SP, TR, M>), dim3(nchains * groups), dim3(UPD_THREADS), 0, s,      \
                        (TS *)tiles, tjjs, ops, chain_start, chain0, groups, per)

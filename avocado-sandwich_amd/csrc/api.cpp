@@ -162,10 +162,11 @@ struct avs_handle {
 namespace {
 
 int resolve_tree(const avs_handle *h) { return h->opts.tree; }
-// 0 = the automatic per-sweep rule of domain_for (two tiles per domain by
-// default: the panel chain of a sweep is then 2 + log2(m/2) panels long, the
-// shortest for m >= 4 up to one, and the update work is the same for every
-// domain size)
+// 0 = the automatic per-sweep rule of domain_for: two tiles per domain where
+// the panel chain bounds a sweep (the chain is then 2 + log2(m/2) panels long,
+// the shortest for m >= 4 up to one), four where its updates do (the update
+// flops are the same for every domain size; larger domains put more of them
+// in the TSQRT updates, the fastest)
 int resolve_domain(const avs_handle *h) { return h->opts.domain > 0 ? h->opts.domain : 0; }
 
 avs_status ensure_program(avs_handle *h, int alg, int kend)

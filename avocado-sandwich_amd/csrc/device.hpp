@@ -937,6 +937,15 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
     }
     __syncthreads();
 
+    // SPW = 1 has the registers to request C1[J] one block ahead (c1nx); SPW
+    // = 2 requests it at the start of block J, before the W MFMAs
+    acc_t c1nx[SPW];
+    if (SPW == 1 && mine && mode != MODE_GE) {
+        const int J0 = APPLYT ? 0 : NB - 1;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) c1nx[0][m] = c1p[0][(long)(16 * J0 + m) * rs1];
+    }
+
 #pragma unroll 1
     for (int jj = 0; jj < NB; ++jj) {
         const int J = APPLYT ? jj : (NB - 1 - jj);
@@ -944,15 +953,23 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
         TS yr[PER];
         if (jj + 1 < NB) yload(APPLYT ? jj + 1 : NB - 2 - jj, yr);
         if (mine) {
-            // C1[J] is requested first and added after the product, so its
-            // load overlaps the W MFMAs
+            // C1[J] is requested before the W MFMAs and added after them
             acc_t c1v[SPW], Wa[SPW];
 #pragma unroll
             for (int sp = 0; sp < SPW; ++sp) {
                 Wa[sp] = vzero4<acc_t>();
                 if (mode != MODE_GE) {
+                    if constexpr (SPW == 1) {
+                        c1v[sp] = c1nx[sp];
+                        if (jj + 1 < NB) {
+                            const int Jn = APPLYT ? jj + 1 : NB - 2 - jj;
 #pragma unroll
-                    for (int m = 0; m < 4; ++m) c1v[sp][m] = c1p[sp][(long)(16 * J + m) * rs1];
+                            for (int m = 0; m < 4; ++m) c1nx[sp][m] = c1p[sp][(long)(16 * Jn + m) * rs1];
+                        }
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) c1v[sp][m] = c1p[sp][(long)(16 * J + m) * rs1];
+                    }
                 }
             }
             // W = Y_J^T C2 + C1[J]

@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(PanelThreads<B>::value) chain_panel_kernel(TS 
 // SPW = 1 (launches too small to fill the GPU, per = 1): two workgroups per
 // column tile, each taking half of its 16-column strips, one strip per wave.
 template <typename TS, int B, bool SPLIT, bool TR, int MODE, int SPW = 2>
-__global__ void __launch_bounds__(UPD_THREADS, 2) chain_update_kernel(TS *tiles, const double *tjjs,
+__global__ void __launch_bounds__(UPD_THREADS, SPW == 1 ? 3 : 2) chain_update_kernel(TS *tiles, const double *tjjs,
                                                                        const ChainOp *__restrict__ ops,
                                                                        const int *__restrict__ chain_start, int chain0,
                                                                        int groups, int per)
@@ -339,7 +339,11 @@ void launch_chain_update(int nchains, int groups, int per, int mode, int trans, 
     // a launch of fewer workgroups than two per CU: half-tile workgroups
     // (b = 128, one column tile per workgroup, native arithmetic)
     if constexpr (B == 128) {
+        // AVSVD_UPD_HALF=all: every such launch; =off: none (read once per process)
         static const int slots = [] {
+            const char *e = getenv("AVSVD_UPD_HALF");
+            if (e && strcmp(e, "all") == 0) return 1 << 30;
+            if (e && strcmp(e, "off") == 0) return 0;
             int dev = 0, ncu = 0;
             if (hipGetDevice(&dev) != hipSuccess ||
                 hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)

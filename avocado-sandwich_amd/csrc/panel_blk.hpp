@@ -332,28 +332,33 @@ __device__ __forceinline__ void panel_blk_body(TS *__restrict__ top, long ldt, T
 #pragma unroll
                 for (int m = 0; m < 4; ++m) W[m] += c1v[m];
             }
-        } else if (w == J && lane < 16) {
-            // T_JJ, one row per lane, while the other strips form W:
-            // T[i][k] = -tau_k sum_{m=i}^{k-1} T[i][m] G[m][k] (T[i][m] = 0 for m < i);
-            // each column of G is read at once and summed in four chains
-            const int il = lane;
-            double trow[16];
+        } else if (w == J) {
+            // T_JJ while the other strips form W, row c in the lanes of column c:
+            // T[c][k] = -tau_k sum_{m=c}^{k-1} T[c][m] G[m][k] (T[c][m] = 0 for m < c).
+            // Each lane reads its own row of G once (gr[k] = G[c][k]); G[m][k]
+            // then comes from lane m by DPP, so the recurrence waits on no LDS
+            double gr[16], trow[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                double g[16], p[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < 16; ++k) gr[k] = GL[k * 16 + c];
+            const double *tb = tauL + 16 * J;
+            static_for<16>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                double p[4] = {0.0, 0.0, 0.0, 0.0};
+                static_for<k>([&](auto mc) {
+                    constexpr int mm = decltype(mc)::value;
+                    p[mm & 3] = fma(trow[mm], row_bcast<mm>(gr[k]), p[mm & 3]);
+                });
+                const double tk = tb[k];
+                trow[k] = c == k ? tk : (c < k ? -tk * ((p[0] + p[1]) + (p[2] + p[3])) : 0.0);
+            });
+            if (q == 0) {
+                double *Tr = TB + (J & 1) * 256 + c * 16;
 #pragma unroll
-                for (int m = 0; m < k; ++m) g[m] = GL[k * 16 + m];
+                for (int k = 0; k < 16; ++k) Tr[k] = trow[k];
+                if (tjj) {
 #pragma unroll
-                for (int m = 0; m < k; ++m) p[m & 3] = fma(trow[m], g[m], p[m & 3]);
-                const double tk = tauL[16 * J + k];
-                trow[k] = il == k ? tk : (il < k ? -tk * ((p[0] + p[1]) + (p[2] + p[3])) : 0.0);
-            }
-            double *Tr = TB + (J & 1) * 256 + il * 16;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) Tr[k] = trow[k];
-            if (tjj) {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) tjj[(16 * J + il) * 16 + k] = trow[k];
+                    for (int k = 0; k < 16; ++k) tjj[(16 * J + c) * 16 + k] = trow[k];
+                }
             }
         }
         AVS_BLK_MARK(5, J);

@@ -29,7 +29,7 @@
 
 namespace avs {
 
-int domain_for(int m, int domain)
+int domain_for(int m, int b, int domain)
 {
     if (domain > 0) return domain;
     // (m_i, d_i) sorted by m_i; parsed once (a function-local static is
@@ -51,10 +51,17 @@ int domain_for(int m, int domain)
                 pos = end + 1;
             }
         }
-        if (r.empty()) r.push_back({0, 2});
         std::sort(r.begin(), r.end());
         return r;
     }();
+    if (rule.empty()) {
+        // a sweep over at least 10240 rows is bound by its updates: four tiles
+        // per domain move merge (TT) and GEQRT update work into the TSQRT
+        // updates, which run at the highest MFMA rate (n=16384 FP64 b=128:
+        // 398-400 ms against 410 ms with two); shorter sweeps are bound by
+        // their panel chain, shortest with two
+        return (long)m * b >= 10240 ? 4 : 2;
+    }
     int d = rule.front().second;
     for (const auto &x : rule)
         if (m >= x.first) d = x.second;
@@ -142,7 +149,7 @@ struct Builder {
         const int cnt = nt - k - 1;
         const int nt2 = nt * nt;
         const int famGE = lq ? 2 : 0, famTT = lq ? 3 : 1;
-        const int h = tree == 0 ? m : domain_for(m, domain);
+        const int h = tree == 0 ? m : domain_for(m, B, domain);
         std::vector<int> heads;
         for (int d0 = 0; d0 < m; d0 += h) {
             heads.push_back(d0);
@@ -335,7 +342,7 @@ struct PhaseBuilder {
         const int cnt = nt - k - 1;
         const int nt2 = nt * nt;
         const int famGE = lq ? 2 : 0, famTT = lq ? 3 : 1;
-        const int h = tree == 0 ? m : domain_for(m, domain);
+        const int h = tree == 0 ? m : domain_for(m, P.B, domain);
         std::vector<int> heads;
         for (int d0 = 0; d0 < m; d0 += h) heads.push_back(d0);
         // GEQRT of every domain head, then its update

@@ -57,14 +57,37 @@ template <int L> __device__ __forceinline__ double row_bcast(double v)
 {
     return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + L, 0xf, 0xf, true);
 }
-// the value of lane (c, QJ) to every lane (c, q): two lane swaps across rows
-// of 16 lanes (rows q ^ 2, then q ^ 1), keeping the side that leads to QJ
-template <int QJ> __device__ __forceinline__ double rowgroup_bcast(double v, int q)
+// Cross-row exchanges with v_permlane{16,32}_swap applied to two copies of
+// one value: permlane16 gives X = rows (0,0,2,2) and Y = rows (1,1,3,3) of
+// rows of 16 lanes, permlane32 gives X = rows (0,1,0,1) and Y = (2,3,2,3).
+struct SwapXY { double x, y; };
+template <int W> __device__ __forceinline__ SwapXY swap_rows(double v)
 {
-    const double x32 = lane_xor<32>(v);
-    v = ((q & 2) == (QJ & 2)) ? v : x32;
-    const double x16 = lane_xor<16>(v);
-    return ((q & 1) == (QJ & 1)) ? v : x16;
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    if constexpr (W == 16) {
+        const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        return {__hiloint2double((int)rh[0], (int)rl[0]), __hiloint2double((int)rh[1], (int)rl[1])};
+    } else {
+        const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        return {__hiloint2double((int)rh[0], (int)rl[0]), __hiloint2double((int)rh[1], (int)rl[1])};
+    }
+}
+// sum over the four row groups (lanes c, c+16, c+32, c+48), the same in every
+// lane (each step adds the same two values in the same order everywhere)
+__device__ __forceinline__ double rowgroup_sum(double v)
+{
+    const SwapXY a = swap_rows<16>(v);
+    const SwapXY b = swap_rows<32>(a.x + a.y);
+    return b.x + b.y;
+}
+// the value of lane (c, QJ) to every lane (c, q)
+template <int QJ> __device__ __forceinline__ double rowgroup_bcast(double v)
+{
+    const SwapXY a = swap_rows<32>(v);
+    const SwapXY b = swap_rows<16>(QJ < 2 ? a.x : a.y);
+    return (QJ & 1) ? b.y : b.x;
 }
 // The four registers of one row block through v_fmac_f64 with the first
 // source read through DPP row_newbcast:L (the pivot lane of each row of 16):
@@ -240,15 +263,13 @@ __device__ __forceinline__ void panel_blk_body(TS *__restrict__ top, long ldt, T
                             pv = a[rb][mj];
                         }
                     });
-                    double S = (sp[0] + sp[1]) + (sp[2] + sp[3]);
-                    S += lane_xor<16>(S);
-                    S += lane_xor<32>(S);
+                    const double S = rowgroup_sum((sp[0] + sp[1]) + (sp[2] + sp[3]));
                     // S_c = x . a_c in every lane of column c; S_jl = ||x||^2
                     const double nn = readlane_d(S, jl);
                     // row j of the factored columns (GE: the tile's row j, TS/TT: R's
                     // row j): alpha its entry in column jl, rowv the entry in this lane's column
                     const double alpha = readlane_d(pv, 16 * qj + jl);
-                    const double rowv = rowgroup_bcast<qj>(pv, q);
+                    const double rowv = rowgroup_bcast<qj>(pv);
                     const bool none = (MODE == MODE_GE && j == B - 1); // GE's last column has no reflector
                     const double n2 = none ? 0.0 : fma(alpha, alpha, nn);
                     const bool nz = n2 != 0.0; // H_j = I for an all-zero column
@@ -264,8 +285,11 @@ __device__ __forceinline__ void panel_blk_body(TS *__restrict__ top, long ldt, T
                     const double tw = taur * wc;
                     // c > jl: a_c -= tau w_c y, R[j][c] -= tau w_c; c == jl: a_jl = x -> x scal
                     // (the stored reflector), R[j][j] = beta; c < jl: finished columns
-                    const double tws = !nz ? 0.0 : (c > jl ? tw * scal : (c == jl ? 1.0 - scal : 0.0));
-                    const double newrow = !nz ? rowv : (c > jl ? rowv - tw : (c == jl ? beta : rowv));
+                    // (as arithmetic on 0/1 lane factors, not as branches)
+                    const double nzf = nz ? 1.0 : 0.0;
+                    const double gt = c > jl ? nzf : 0.0, eq = c == jl ? nzf : 0.0;
+                    const double tws = fma(gt, tw * scal, eq * (1.0 - scal));
+                    const double newrow = (c == jl && nz) ? beta : fma(-gt, tw, rowv);
                     // a_c -= tws x, x broadcast again from the pivot column (its
                     // lanes read their own value before they update it)
                     const double ntws = -tws;

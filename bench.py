@@ -104,8 +104,8 @@ def main():
     ap.add_argument("--sched", choices=["streams", "host"], default="streams")
     ap.add_argument("--tree", choices=["hier", "flat"], default="hier")
     ap.add_argument("--domain", type=int, default=0)
-    ap.add_argument("--reserve-cus", type=int, default=16,
-                    help="CUs (the same number per XCD) the CU-masked update stream leaves to the panels")
+    ap.add_argument("--reserve-cus", type=int, default=0,
+                    help="CUs (the same number per XCD) a CU-masked update stream leaves to the panels (0: none)")
     ap.add_argument("--masked-modes", type=int, default=2,
                     help="update modes on the CU-masked stream: bit 0 GEQRT's, 1 TSQRT's, 2 merges' (0 = all)")
     ap.add_argument("--cpu-ns", type=int, nargs="+", default=[1024, 4096])
@@ -153,11 +153,11 @@ def main():
     o.wait_ms = 120000
     o.phase_timing = 1 if args.sched == "streams" else 0  # per-launch events for the roofline
     if args.sched == "streams":
-        # the TSQRT updates on a stream that leaves reserve_cus CUs to the
-        # panels, so that the first merge panels of a sweep start at once
-        # (n=16384 FP64 b=128: 452-453 ms against 467-469 ms with every update
-        # on all CUs and 459-460 ms with the merge updates masked too;
-        # bit-identical bands)
+        # optionally the updates of some modes on a stream that leaves
+        # reserve_cus CUs to the panels (bit-identical bands).  With the
+        # blocked panel every update on all CUs is fastest at n=16384 FP64
+        # b=128: 391.6 ms against 399.5-400.1 ms with 8-24 CUs left to the
+        # panels beside the TSQRT updates
         o.reserve_cus = args.reserve_cus
         o.masked_modes = args.masked_modes
     h = ctypes.c_void_p()

@@ -235,33 +235,32 @@ __device__ __forceinline__ void panel_blk_body(TS *__restrict__ top, long ldt, T
             // straight-line instance per J (JS), selected once per strip; TS
             // touches every block and has one instance (JS = -1).
             auto factor = [&](auto) {
-                // row block rb: 0 untouched, 1 every row, 2 GE's diagonal block (rows below j)
+                // row blocks the reflectors touch (1) or not (0)
                 auto blk = [](int rb) constexpr {
-                    return MODE == MODE_TS ? 1
-                         : MODE == MODE_TT ? (rb <= JS ? 1 : 0)
-                         : (rb > JS ? 1 : (rb == JS ? 2 : 0));
+                    return MODE == MODE_TS ? 1 : MODE == MODE_TT ? (rb <= JS ? 1 : 0) : (rb >= JS ? 1 : 0);
                 };
                 static_for<16>([&](auto jc) {
                     constexpr int jl = decltype(jc)::value;
                     constexpr int qj = jl / 4, mj = jl % 4;
                     const int j = 16 * J + jl;
                     d4_t sp = {0.0, 0.0, 0.0, 0.0};
-                    double pv = (MODE == MODE_GE) ? 0.0 : rd[mj]; // row j of this lane's column (row group qj)
+                    // row j of this lane's column (row group qj): TS/TT R's row j;
+                    // GE the tile's row j, taken out of the columns c >= jl of the
+                    // strip (so column jl is zero on rows <= j and x needs no row
+                    // mask; the finished columns keep their reflector entry in
+                    // row j; the R rows collect in rd and go back at the end)
+                    double pv;
+                    if constexpr (MODE == MODE_GE) {
+                        pv = a[JS][mj];
+                        if (q == qj && c >= jl) a[JS][mj] = 0.0;
+                    } else {
+                        pv = rd[mj];
+                    }
                     // EXEC writes of the previous reflector: 5 wait states before a DPP
                     asm volatile("s_nop 4");
                     static_for<NB>([&](auto rbc) {
                         constexpr int rb = decltype(rbc)::value;
-                        if constexpr (blk(rb) == 1) {
-                            sp = dot4<jl>(sp, a[rb]);
-                        } else if constexpr (blk(rb) == 2) {
-                            // x = column jl below row j: the mask is applied to this
-                            // lane's own operand (the same row)
-                            d4_t y;
-#pragma unroll
-                            for (int m = 0; m < 4; ++m) y[m] = (4 * q + m > jl) ? a[rb][m] : 0.0;
-                            sp = dot4<jl>(sp, a[rb], y);
-                            pv = a[rb][mj];
-                        }
+                        if constexpr (blk(rb) != 0) sp = dot4<jl>(sp, a[rb]);
                     });
                     const double S = rowgroup_sum((sp[0] + sp[1]) + (sp[2] + sp[3]));
                     // S_c = x . a_c in every lane of column c; S_jl = ||x||^2
@@ -296,17 +295,9 @@ __device__ __forceinline__ void panel_blk_body(TS *__restrict__ top, long ldt, T
                     asm volatile("s_nop 4");
                     static_for<NB>([&](auto rbc) {
                         constexpr int rb = decltype(rbc)::value;
-                        if constexpr (blk(rb) == 1) {
-                            a[rb] = axpy4<jl>(a[rb], ntws);
-                        } else if constexpr (blk(rb) == 2) {
-                            d4_t sm;
-#pragma unroll
-                            for (int m = 0; m < 4; ++m) sm[m] = (4 * q + m > jl) ? ntws : 0.0;
-                            a[rb] = axpy4<jl>(a[rb], sm);
-                            if (q == qj && c >= jl) a[rb][mj] = newrow;
-                        }
+                        if constexpr (blk(rb) != 0) a[rb] = axpy4<jl>(a[rb], ntws);
                     });
-                    if (MODE != MODE_GE && q == qj && c >= jl) rd[mj] = newrow;
+                    if (q == qj && c >= jl) rd[mj] = newrow;
                     // one LDS store: the Gram entry (c < jl) or tau_j (c == jl)
                     if (q == 0 && c <= jl) {
                         double *dst = c < jl ? GL + jl * 16 + c : tauL + j;
@@ -315,6 +306,13 @@ __device__ __forceinline__ void panel_blk_body(TS *__restrict__ top, long ldt, T
                 });
             };
             factor(jsc);
+            if constexpr (MODE == MODE_GE) {
+                // the strip's diagonal block: R (rows <= column, in rd) and the
+                // reflectors (rows > column, in the strip) have disjoint supports
+#pragma unroll
+                for (int m = 0; m < 4; ++m) a[JS][m] += rd[m];
+                rd = d4_t{0.0, 0.0, 0.0, 0.0};
+            }
             AVS_BLK_MARK(2, J);
             // ---- Y_J for the MFMA updates (reflector rows with the implicit
             // unit pivot and zeros written out)

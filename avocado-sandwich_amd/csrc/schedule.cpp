@@ -55,12 +55,14 @@ int domain_for(int m, int b, int domain)
         return r;
     }();
     if (rule.empty()) {
-        // a sweep over at least 10240 rows is bound by its updates: four tiles
-        // per domain move merge (TT) and GEQRT update work into the TSQRT
-        // updates, which run at the highest MFMA rate (n=16384 FP64 b=128:
-        // 398-400 ms against 410 ms with two); shorter sweeps are bound by
-        // their panel chain, shortest with two
-        return (long)m * b >= 10240 ? 4 : 2;
+        // a long sweep is bound by its updates: larger domains move merge (TT)
+        // and GEQRT update work into the TSQRT updates, which run at the
+        // highest MFMA rate; short sweeps are bound by their panel chain,
+        // shortest with two tiles per domain (n=16384 FP64 b=128: 376 ms with
+        // 4 from 10240 rows and 3 from 5120, 379 ms with 4 from 10240, 397 ms
+        // with 2 throughout)
+        const long rows = (long)m * b;
+        return rows >= 10240 ? 4 : (rows >= 5120 ? 3 : 2);
     }
     int d = rule.front().second;
     for (const auto &x : rule)

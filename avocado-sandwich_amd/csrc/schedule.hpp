@@ -42,7 +42,7 @@ void build_phase_program(PhaseProgram &P, int nt, int B, int alg, int kend, int 
 
 // Tiles per domain of the hierarchical tree for a sweep over m panel tiles of
 // size b: `domain` when it is positive, otherwise the automatic rule (4 when
-// m*b >= 10240, else 2; or the rule in AVSVD_DOMAIN_RULE = "m0:d0,m1:d1,...":
+// m*b >= 10240, 3 when m*b >= 5120, else 2; or the rule in AVSVD_DOMAIN_RULE = "m0:d0,m1:d1,...":
 // d_i for m >= m_i, the largest such m_i applying; read once per process).
 int domain_for(int m, int b, int domain);
 

@@ -25,7 +25,8 @@ EXPORTS = (
     "avs_svdvals", "avs_synchronize", "avs_last_time_ms", "avs_last_error", "avs_bidiagonalize_band",
     "avs_bidiagonalize_band_lapack", "avs_bidiagonalize_band_chase", "avs_bidiagonalize_band_device",
     "avs_bdsdc_values", "avs_geqrt", "avs_tsqrt", "avs_unmqr", "avs_tsmqr", "avs_tsmqr_noalloc", "avs_gen_gaussian_host",
-    "avs_debug_mfma_layout", "avs_schedule_stats", "avs_schedule_dump", "avs_phase_dump", "avs_phase_timing", "avs_phase_trace",
+    "avs_debug_mfma_layout", "avs_schedule_stats", "avs_schedule_dump", "avs_phase_dump", "avs_phase_timing", "avs_set_phase_timing",
+    "avs_phase_trace",
     "avs_batch_create", "avs_batch_destroy", "avs_batch_set_matrix_host", "avs_batch_blockbidiag",
     "avs_batch_last_time_ms", "avs_batch_get_band", "avs_batch_svdvals", "avs_set_matrix_host_sub",
     "avs_server_default_opts", "avs_server_create", "avs_server_submit", "avs_server_poll", "avs_server_wait",
@@ -111,6 +112,7 @@ def lib():
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)],
             "avs_phase_dump": [i32, i32, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int32), i64,
                                ctypes.POINTER(i64)],
+            "avs_set_phase_timing": [vp, i32],
             "avs_phase_trace": [vp, dp, dp, ctypes.POINTER(ctypes.c_int32), i64, ctypes.POINTER(i64)],
             "avs_batch_create": [ctypes.POINTER(vp), i32, i32, ctypes.POINTER(i64), i32, ctypes.POINTER(Opts)],
             "avs_batch_destroy": [vp],

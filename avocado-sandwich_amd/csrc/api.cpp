@@ -152,6 +152,7 @@ struct avs_handle {
     // panel stream of the stream-ordered executor and one completion event per
     // phase (the panels of a sweep run ahead of that sweep's updates)
     hipStream_t pstream = nullptr;
+    int pslot = -1;  // the panel stream's slot in its device's pool
     std::vector<hipEvent_t> done;
     // update stream whose CU mask leaves opts.reserve_cus CUs to the panels,
     // and the events that join it to the handle's stream
@@ -160,6 +161,61 @@ struct avs_handle {
 };
 
 namespace {
+
+// Panel streams come from a small per-device pool of streams that each own a
+// hardware queue.  The HIP runtime maps ordinary streams onto a few shared
+// hardware queues (GPU_MAX_HW_QUEUES, 4 by default); once a process holds more
+// streams than that, a handle's panel and update streams can land on one
+// queue, where every panel waits behind the updates enqueued before it (n=16384
+// FP64 b=128 on MI355X: 474 ms instead of 358 ms for the handle concerned, the
+// one-stream time).  A stream created with a CU mask gets a queue of its own;
+// the mask here enables every CU.  At most kPanelQueues such streams per
+// device: with a sixth next to the runtime's four queues its handle was slower
+// again (428 ms).  Further handles share the pool's streams, least used first
+// (their panels then serialise with another handle's panels, never with their
+// own updates).  The pool's streams live until the process ends.
+constexpr int kPanelQueues = 4;
+struct PanelPool {
+    std::vector<hipStream_t> s;
+    std::vector<int> users;
+};
+std::mutex g_ppool_mu;
+std::map<int, PanelPool> g_ppool;
+
+avs_status acquire_panel_stream(avs_handle *h)
+{
+    if (h->pstream) return AVS_OK;
+    std::lock_guard<std::mutex> lk(g_ppool_mu);
+    PanelPool &P = g_ppool[h->device];
+    int slot = -1;
+    for (size_t i = 0; i < P.s.size(); ++i)
+        if (slot < 0 || P.users[i] < P.users[slot]) slot = (int)i;
+    if ((slot < 0 || P.users[slot] > 0) && (int)P.s.size() < kPanelQueues) {
+        int ncu = 0;
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+        // the size argument counts 32-bit words; ncu words covers every CU
+        std::vector<uint32_t> mask(ncu, 0);
+        for (int i = 0; i < ncu; ++i) mask[i / 32] |= 1u << (i % 32);
+        hipStream_t st = nullptr;
+        HIPCHK(hipExtStreamCreateWithCUMask(&st, (uint32_t)ncu, mask.data()));
+        P.s.push_back(st);
+        P.users.push_back(0);
+        slot = (int)P.s.size() - 1;
+    }
+    ++P.users[slot];
+    h->pstream = P.s[slot];
+    h->pslot = slot;
+    return AVS_OK;
+}
+
+void release_panel_stream(avs_handle *h)
+{
+    if (h->pslot < 0) return;
+    std::lock_guard<std::mutex> lk(g_ppool_mu);
+    --g_ppool[h->device].users[h->pslot];
+    h->pslot = -1;
+    h->pstream = nullptr;
+}
 
 int resolve_tree(const avs_handle *h) { return h->opts.tree; }
 // 0 = the automatic per-sweep rule of domain_for: two tiles per domain where
@@ -305,7 +361,7 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
         h->pflops.assign(nph, 0.0);
     }
     if (two) {
-        if (!h->pstream) HIPCHK(hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking));
+        if (avs_status st = acquire_panel_stream(h)) return st;
         while (h->done.size() < nph + 1) {
             hipEvent_t e;
             HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -412,6 +468,9 @@ template <typename TS, int B> avs_status run_streams_graph(avs_handle *h)
     }
     if (!h->gexec) {
         const size_t nph = h->phases->P.phases.size();
+        // a capture joins the panel stream to the graph: it must be the
+        // handle's own (a pooled stream may carry other handles' launches)
+        if (h->pslot >= 0) release_panel_stream(h);
         if (!h->pstream) HIPCHK(hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking));
         while (h->done.size() < nph + 1) {
             hipEvent_t e;
@@ -755,7 +814,10 @@ avs_status avs_destroy(avs_handle *h)
     if (h->ev1) hipEventDestroy(h->ev1);
     for (hipEvent_t e : h->pev) hipEventDestroy(e);
     for (hipEvent_t e : h->done) hipEventDestroy(e);
-    if (h->pstream) hipStreamDestroy(h->pstream);
+    // a pooled panel stream goes back to its device's pool (the handle's
+    // stream, synchronised above, waited for its last panel)
+    if (h->pslot >= 0) release_panel_stream(h);
+    else if (h->pstream) hipStreamDestroy(h->pstream);
     if (h->ustream) hipStreamDestroy(h->ustream);
     for (hipEvent_t e : h->ujoin)
         if (e) hipEventDestroy(e);
@@ -939,6 +1001,13 @@ avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_laun
     if (update_launches) *update_launches = un;
     if (update_flops) *update_flops = uf;
     return check_status(h);
+}
+
+avs_status avs_set_phase_timing(avs_handle *h, int32_t on)
+{
+    if (!h) return fail(-1, "null handle");
+    h->opts.phase_timing = on != 0;
+    return AVS_OK;
 }
 
 avs_status avs_phase_trace(avs_handle *h, double *start_ms, double *end_ms, int32_t *kind_mode, int64_t cap,

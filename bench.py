@@ -96,7 +96,7 @@ def panel_rate(n, b, tree, domain, dtype, panel_ms):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--b", type=int, default=128)
@@ -151,7 +151,10 @@ def main():
     o.tree = avsvd.TREE_HIER if args.tree == "hier" else avsvd.TREE_FLAT
     o.domain = args.domain
     o.wait_ms = 120000
-    o.phase_timing = 1 if args.sched == "streams" else 0  # per-launch events for the roofline
+    # per-launch timing events (for the roofline) only in the last timed step:
+    # they add launch latency to the panel chain (n=16384 FP64 b=128: 374 vs
+    # 356 ms per reduction)
+    o.phase_timing = 0
     if args.sched == "streams":
         # optionally the updates of some modes on a stream that leaves
         # reserve_cus CUs to the panels (bit-identical bands).  With the
@@ -182,6 +185,8 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
+        if s == args.steps - 1 and args.sched == "streams":
+            check(L.avs_set_phase_timing(h, 1))
         step()
         check(L.avs_last_time_ms(h, ctypes.byref(ms)))  # waits for this step's kernels
         kernel_ms.append(ms.value)
@@ -191,9 +196,9 @@ def main():
     for s, k in enumerate(kernel_ms):
         log("step %d: %.1f ms" % (s, k))
     if args.sched == "streams":
-        # per-launch HIP events of the last timed step (recorded in every timed
-        # step; read after the timed region, so the host-side readout of
-        # thousands of events is not counted in the step time)
+        # per-launch HIP events of the last timed step (the only step that
+        # records them; read after the timed region, so the host-side readout
+        # of thousands of events is not counted in the step time)
         pm, um, uf = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         pn, un = ctypes.c_int64(), ctypes.c_int64()
         check(L.avs_phase_timing(h, ctypes.byref(pm), ctypes.byref(pn), ctypes.byref(um), ctypes.byref(un),
@@ -319,7 +324,8 @@ def main():
                      "launches": upd["launches"], "kernel_share_of_step": round(upd["ms"] / kernel_ms[-1], 4)
                      if upd["launches"] else None,
                      "panel_ms_per_step": round(upd["panel_ms"], 3),
-                     "step_kernel_ms": [round(x, 3) for x in kernel_ms]},
+                     "step_kernel_ms": [round(x, 3) for x in kernel_ms],
+                     "launch_events": "per-launch HIP events on each launch's own stream, last timed step only"},
         "panels": panel_rate(n, b, o.tree, args.domain, args.dtype, upd["panel_ms"]),
         "band_fro_rel_err": fro_err,
         "sigma_check": sigma_check,

@@ -165,6 +165,12 @@ avs_status avs_ooc_tiled_qr(int32_t dtype, int64_t n, int32_t b, void *A, int64_
  * the run.  No counterpart in the reference (its benchmarks time whole calls). */
 avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_launches, double *update_ms,
                             int64_t *update_launches, double *update_flops);
+/* Turn the per-launch timing events of opts.phase_timing on (on != 0) or off
+ * for the handle's next runs.  The events cost launch latency on the panel
+ * chain (n=16384 FP64 b=128: 374 vs 356 ms per reduction), so a caller that
+ * times several runs can record them in one run only.  No counterpart in the
+ * reference. */
+avs_status avs_set_phase_timing(avs_handle *h, int32_t on);
 /* Per-launch timeline of the last timed stream-ordered run (opts.phase_timing):
  * for launch i, start and end in ms from the first launch's start event, and
  * kind*4 + mode (kind 0 panel, 1 update; mode 0 GE, 1 TS, 2 TT).  With null

@@ -840,6 +840,11 @@ __device__ __forceinline__ void update_body_split(const float *__restrict__ V, l
 // Handles up to 2*NT/64 strips (ntiles * B/16 of them) per call.
 // ---------------------------------------------------------------------------
 constexpr int UPD_THREADS = 256;
+// timing hook of update2_body (event ev at block jj); empty unless a probe
+// build defines it
+#ifndef AVS_UPD_MARK
+#define AVS_UPD_MARK(ev, jj)
+#endif
 template <typename TS, int B> struct Lds2 {
     static constexpr int YS = 17; // row stride of a staged 16-reflector block
     static constexpr int upd = (2 * B * YS * (int)sizeof(TS) + 16 * B * (int)sizeof(TS) + 7) / 8;
@@ -866,6 +871,7 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
     // perm(d) = 4*(d % 4) + d / 4 (the identity for the f32 layout, whose
     // hardware row already is 4q + m).
     auto perm = [](int d) { return sizeof(TS) == 8 ? 4 * (d & 3) + (d >> 2) : d; };
+    AVS_UPD_MARK(0, 0);
     constexpr int PER = (B * 16) / NT; // staged block elements per thread
     static_assert((B * 16) % NT == 0 && PER >= 1, "update2 staging split");
     TS *YB = reinterpret_cast<TS *>(smem); // YB[buf*B*YS + k*YS + i] = Y[k][16J + i]
@@ -936,6 +942,7 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
         }
     }
     __syncthreads();
+    AVS_UPD_MARK(1, 0);
 
     // SPW = 1 has the registers to request C1[J] one block ahead (c1nx); SPW
     // = 2 requests it at the start of block J, before the W MFMAs
@@ -984,6 +991,7 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
                     for (int sp = 0; sp < SPW; ++sp) Wa[sp] = mfma16(av, acc2[sp][tb][m], Wa[sp]);
                 }
             }
+            AVS_UPD_MARK(2, jj);
             if (mode != MODE_GE) {
 #pragma unroll
                 for (int sp = 0; sp < SPW; ++sp) {
@@ -991,6 +999,7 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
                     for (int m = 0; m < 4; ++m) Wa[sp][m] += c1v[sp][m];
                 }
             }
+            AVS_UPD_MARK(3, jj);
             // W' = T_JJ^T W (Q^T) or T_JJ W (Q)
             acc_t Wp[SPW];
 #pragma unroll
@@ -1008,6 +1017,7 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
                     for (int m = 0; m < 4; ++m) c1p[sp][(long)(16 * J + m) * rs1] = c1v[sp][m] - Wp[sp][m];
                 }
             }
+            AVS_UPD_MARK(4, jj);
             // C2 -= Y_J W'
 #pragma unroll
             for (int tb = 0; tb < NB; ++tb) {
@@ -1021,8 +1031,11 @@ __device__ __forceinline__ void update2_body(int mode, const TS *__restrict__ V,
                 }
             }
         }
+        AVS_UPD_MARK(5, jj);
         if (jj + 1 < NB) ystore((jj + 1) & 1, yr);
+        AVS_UPD_MARK(6, jj);
         __syncthreads();
+        AVS_UPD_MARK(7, jj);
     }
     if (mine) {
 #pragma unroll

@@ -124,6 +124,28 @@ template <int L> __device__ __forceinline__ d4_t dot4(d4_t acc, d4_t x)
                  : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "n"(L));
     return d4_t{a0, a1, a2, a3};
 }
+// p_i += x[lane L0 + i of the row] * t_i for i = 0..3: four terms of four
+// independent sums, one fused DPP FMA each (the T_JJ recurrence)
+template <int L0>
+__device__ __forceinline__ void fmac_bc4(double &p0, double &p1, double &p2, double &p3, double x, double t0, double t1,
+                                         double t2, double t3)
+{
+    asm volatile("s_nop 1\n\t"
+                 "v_fmac_f64_dpp %0, %4, %5 row_newbcast:%9 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+                 "v_fmac_f64_dpp %1, %4, %6 row_newbcast:%10 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+                 "v_fmac_f64_dpp %2, %4, %7 row_newbcast:%11 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+                 "v_fmac_f64_dpp %3, %4, %8 row_newbcast:%12 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3)
+                 : "v"(x), "v"(t0), "v"(t1), "v"(t2), "v"(t3), "n"(L0), "n"(L0 + 1), "n"(L0 + 2), "n"(L0 + 3));
+}
+// p += x[lane L of the row] * t
+template <int L> __device__ __forceinline__ void fmac_bc1(double &p, double x, double t)
+{
+    asm volatile("s_nop 1\n\t"
+                 "v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "+v"(p)
+                 : "v"(x), "v"(t), "n"(L));
+}
 // axpy4 with one s for the four registers
 template <int L> __device__ __forceinline__ d4_t axpy4(d4_t a, double s)
 {
@@ -359,19 +381,29 @@ __device__ __forceinline__ void panel_blk_body(TS *__restrict__ top, long ldt, T
             // T[c][k] = -tau_k sum_{m=c}^{k-1} T[c][m] G[m][k] (T[c][m] = 0 for m < c).
             // Each lane reads its own row of G once (gr[k] = G[c][k]); G[m][k]
             // then comes from lane m by DPP, so the recurrence waits on no LDS
+            // (term m of row c's sum goes to partial sum m % 4; one fused DPP
+            // FMA per term)
             double gr[16], trow[16];
 #pragma unroll
             for (int k = 0; k < 16; ++k) gr[k] = GL[k * 16 + c];
             const double *tb = tauL + 16 * J;
+            asm volatile("s_nop 4"); // EXEC written by the branch above: 5 wait states before a DPP read
             static_for<16>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                double p[4] = {0.0, 0.0, 0.0, 0.0};
-                static_for<k>([&](auto mc) {
-                    constexpr int mm = decltype(mc)::value;
-                    p[mm & 3] = fma(trow[mm], row_bcast<mm>(gr[k]), p[mm & 3]);
+                double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+                static_for<k / 4>([&](auto gc) {
+                    constexpr int g4 = 4 * decltype(gc)::value;
+                    fmac_bc4<g4>(p0, p1, p2, p3, gr[k], trow[g4], trow[g4 + 1], trow[g4 + 2], trow[g4 + 3]);
+                });
+                static_for<k % 4>([&](auto rc) {
+                    constexpr int mm = 4 * (k / 4) + decltype(rc)::value;
+                    if constexpr ((mm & 3) == 0) fmac_bc1<mm>(p0, gr[k], trow[mm]);
+                    else if constexpr ((mm & 3) == 1) fmac_bc1<mm>(p1, gr[k], trow[mm]);
+                    else if constexpr ((mm & 3) == 2) fmac_bc1<mm>(p2, gr[k], trow[mm]);
+                    else fmac_bc1<mm>(p3, gr[k], trow[mm]);
                 });
                 const double tk = tb[k];
-                trow[k] = c == k ? tk : (c < k ? -tk * ((p[0] + p[1]) + (p[2] + p[3])) : 0.0);
+                trow[k] = c == k ? tk : (c < k ? -tk * ((p0 + p1) + (p2 + p3)) : 0.0);
             });
             if (q == 0) {
                 double *Tr = TB + (J & 1) * 256 + c * 16;

@@ -199,6 +199,43 @@ uint64_t admit(avs_server *S, const Request &r, std::vector<Slot> &evict)
     return x.last_use;
 }
 
+// The pending request admitted next: the oldest, or with
+// AVSVD_SERVER_ORDER=largest the one of largest padded order (ties: the
+// oldest).  Largest-first did not pay on 24 mixed requests (9.6 solves/s at
+// saturation with either order, warm pool; 9.7 vs 10.1 cold).  Called with
+// S->mu held.
+size_t next_pending(const avs_server *S)
+{
+    static const bool largest = [] {
+        const char *e = getenv("AVSVD_SERVER_ORDER");
+        return e && strcmp(e, "largest") == 0;
+    }();
+    size_t pick = 0;
+    if (largest)
+        for (size_t i = 1; i < S->pending.size(); ++i)
+            if (S->pending[i]->N > S->pending[pick]->N) pick = i;
+    return pick;
+}
+
+// memcpy split over up to 8 threads for large copies (the pinned staging copy
+// of a large request runs on the device thread)
+void copy_parallel(void *dst, const void *src, size_t bytes)
+{
+    const size_t chunk = (size_t)32 << 20;
+    const int nt = (int)std::min<size_t>(8, bytes / chunk);
+    if (nt <= 1) {
+        memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = (bytes / nt + 4095) & ~(size_t)4095;
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) {
+        const size_t a = std::min(bytes, per * i), e = std::min(bytes, per * (i + 1));
+        if (a < e) th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, e - a); });
+    }
+    for (std::thread &t : th) t.join();
+}
+
 Slot *slot_of(avs_server *S, uint64_t key)
 {
     for (Slot &x : S->pool)
@@ -277,7 +314,7 @@ avs_status start(avs_server *S, Slot &x, Running &run)
             return fail(AVS_ERR_ALLOC, "server: pinned staging of %zu bytes", es * (size_t)r.N * r.N);
     }
     // the slot is idle, so its previous upload from hpin has completed
-    memcpy(x.hpin, r.A.data(), r.A.size());
+    copy_parallel(x.hpin, r.A.data(), r.A.size());
     if ((st = avs_set_matrix_host_sub(x.h, x.hpin, r.n, r.n))) return st;
     if ((st = avs_blockbidiag(x.h, 0))) return st;
     if ((st = avs::band_enqueue(x.h, &run.band))) return st;
@@ -321,13 +358,14 @@ void device_thread(avs_server *S)
             }
             stop = S->stop;
             while (!stop && !S->pending.empty() && (int)(running.size() + starting.size()) < S->opts.max_inflight) {
-                const uint64_t key = admit(S, *S->pending.front(), evict);
+                const size_t pick = next_pending(S);
+                const uint64_t key = admit(S, *S->pending[pick], evict);
                 if (!key) break;
                 Running run;
-                run.r = S->pending.front();
+                run.r = S->pending[pick];
                 run.key = key;
                 run.r->t_admit = now_ms();
-                S->pending.pop_front();
+                S->pending.erase(S->pending.begin() + (long)pick);
                 starting.push_back(run);
             }
             for (Bisect &q : bis)
@@ -466,12 +504,20 @@ void host_worker(avs_server *S)
         const int64_t N = r->N;
         r->d.assign(N, 0.0);
         r->e.assign(std::max<int64_t>(N - 1, 1), 0.0);
-        // one host thread per request, four from n = 4096 (the largest
-        // requests set the p99; the sweeps pipeline well there).  Every host
-        // thread from n = 8192 was measured slower at saturation: the
-        // overlapping large requests then oversubscribe the cores.
+        // one host thread per request, four from n = 4096 and
+        // AVSVD_SERVER_BIG_THREADS (default 8) from n = 8192: the largest
+        // requests set the p99 and the makespan, and their sweeps pipeline
+        // well.  24 mixed requests on a 16-core host: 8 threads gave Poisson
+        // p99 1.34 s against 1.69 s with 4, and 9.9 against 9.6 solves/s at
+        // saturation; every host thread from n = 8192 was slower (the
+        // overlapping large requests then oversubscribe the cores).
+        static const int big_threads = [] {
+            const char *e = getenv("AVSVD_SERVER_BIG_THREADS");
+            const int v = e ? atoi(e) : 0;
+            return v > 0 ? v : 8;
+        }();
         avs_status st = avs::bidiagonalize_band_one(N, r->b, r->band.data(), r->b + 1, r->d.data(), r->e.data(),
-                                                    N >= 4096 ? 4 : 1);
+                                                    N >= 8192 ? big_threads : (N >= 4096 ? 4 : 1));
         std::vector<double>().swap(r->band);
         if (!st && dev_sigma && N <= S->nmax && avs::bisect_on_device(N)) {
             // the bisection runs on the device thread

@@ -120,6 +120,56 @@ def test_panel_stream_overlap_is_bitwise_neutral(gpu, n, b, dt, tree):
     np.testing.assert_array_equal(outs[0], outs[1])
 
 
+def test_concurrent_handles_share_panel_streams_bitwise(gpu):
+    """Six handles reduced at once (every run enqueued before any is waited
+    for): panel streams come from a per-device pool of four streams with their
+    own hardware queues, so two pairs of handles share one; every band equals
+    the same matrix reduced by its handle alone."""
+    import ctypes
+    from avsvd._lib import Opts, check, dptr, lib
+    L = lib()
+    n, b, k = 512, 64, 6
+    mats = [np.asfortranarray(O.gaussian_matrix(n, BASE + 31 + i)) for i in range(k)]
+    o = Opts()
+    L.avs_default_opts(ctypes.byref(o))
+    hs = []
+    try:
+        for _ in range(k):
+            h = ctypes.c_void_p()
+            check(L.avs_create(ctypes.byref(h), avsvd.AVS_F64, n, b, ctypes.byref(o)))
+            hs.append(h)
+
+        def bands(together):
+            print("case n=%d b=%d handles=%d together=%s" % (n, b, k, together), flush=True)
+            for h, a in zip(hs, mats):
+                check(L.avs_set_matrix_host(h, a.ctypes.data_as(ctypes.c_void_p), n))
+                check(L.avs_synchronize(h))
+            for h in hs:
+                check(L.avs_blockbidiag(h, 0))
+                if not together:
+                    check(L.avs_synchronize(h))
+            out = []
+            for h in hs:
+                band = np.zeros((b + 1, n), order="F")
+                check(L.avs_get_band(h, dptr(band), b + 1))
+                out.append(band)
+            return out
+
+        alone = bands(False)
+        together = bands(True)
+    finally:
+        for h in hs:
+            L.avs_destroy(h)
+    for x, y in zip(alone, together):
+        np.testing.assert_array_equal(x, y)
+    s = np.linalg.svd(mats[0], compute_uv=False)
+    full = np.zeros((n, n))
+    for d in range(b + 1):
+        idx = np.arange(n - d)
+        full[idx, idx + d] = alone[0][b - d, idx + d]
+    assert np.max(np.abs(np.linalg.svd(full, compute_uv=False) - s)) <= bar(np.float64, n) * s[0]
+
+
 def test_reserved_cus_are_bitwise_neutral(gpu):
     """Updates on a CU-masked stream (16 CUs left to the panels), all of them
     or only the TSQRT updates (the bench configuration), the merges or both, give the same bits as the default

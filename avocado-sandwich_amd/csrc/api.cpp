@@ -145,6 +145,7 @@ struct avs_handle {
     std::vector<int> pkind, pmode;
     std::vector<double> pflops;
     int plaunches = 0;
+    bool ptimed = false; // the last stream-ordered run recorded per-launch events
     // the stream-ordered run captured once as a HIP graph and replayed, and
     // the phase program it was captured from
     hipGraphExec_t gexec = nullptr;
@@ -350,6 +351,7 @@ template <typename TS, int B> avs_status run_streams(avs_handle *h)
     const bool two = h->opts.single_stream == 0;
     const size_t nph = D.P.phases.size();
     h->plaunches = 0;
+    h->ptimed = timing;
     if (timing) {
         while (h->pev.size() < 2 * nph) {
             hipEvent_t e;
@@ -984,8 +986,8 @@ avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_laun
                             int64_t *update_launches, double *update_flops)
 {
     if (!h) return fail(-1, "null handle");
-    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.phase_timing == 0 || h->plaunches == 0)
-        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.phase_timing = 1)");
+    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || !h->ptimed || h->plaunches == 0)
+        return fail(AVS_ERR_STATE, "the last run recorded no per-launch events (opts.phase_timing = 1 or avs_set_phase_timing before it)");
     HIPCHK(hipStreamSynchronize(h->stream));
     double pm = 0, um = 0, uf = 0;
     int64_t pn = 0, un = 0;
@@ -1014,8 +1016,8 @@ avs_status avs_phase_trace(avs_handle *h, double *start_ms, double *end_ms, int3
                            int64_t *count)
 {
     if (!h) return fail(-1, "null handle");
-    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || h->opts.phase_timing == 0 || h->plaunches == 0)
-        return fail(AVS_ERR_STATE, "no timed stream-ordered run (create the handle with opts.phase_timing = 1)");
+    if (!h->timed || h->opts.sched != AVS_SCHED_STREAMS || !h->ptimed || h->plaunches == 0)
+        return fail(AVS_ERR_STATE, "the last run recorded no per-launch events (opts.phase_timing = 1 or avs_set_phase_timing before it)");
     if (count) *count = h->plaunches;
     if (!start_ms || !end_ms || !kind_mode) return AVS_OK;
     if (cap < h->plaunches) return fail(-5, "cap < number of launches");

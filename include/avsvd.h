@@ -156,7 +156,8 @@ avs_status avs_ooc_tiled_qr(int32_t dtype, int64_t n, int32_t b, void *A, int64_
                             const avs_opts *opts);
 
 /* Per-launch device time of the last avs_blockbidiag / avs_tiled_qr run under
- * AVS_SCHED_STREAMS, for handles created with opts.phase_timing != 0 (one event
+ * AVS_SCHED_STREAMS, if that run recorded per-launch events (opts.phase_timing
+ * != 0, or avs_set_phase_timing before it; AVS_ERR_STATE otherwise) (one event
  * pair per launch, on the stream the launch runs on; with the default second
  * panel stream, panel and update launches overlap in time, so the two sums can
  * exceed the run's wall time): summed milliseconds and launch counts of the panel and
@@ -171,7 +172,8 @@ avs_status avs_phase_timing(avs_handle *h, double *panel_ms, int64_t *panel_laun
  * times several runs can record them in one run only.  No counterpart in the
  * reference. */
 avs_status avs_set_phase_timing(avs_handle *h, int32_t on);
-/* Per-launch timeline of the last timed stream-ordered run (opts.phase_timing):
+/* Per-launch timeline of the last stream-ordered run, if it recorded per-launch
+ * events (as for avs_phase_timing):
  * for launch i, start and end in ms from the first launch's start event, and
  * kind*4 + mode (kind 0 panel, 1 update; mode 0 GE, 1 TS, 2 TT).  With null
  * arrays only *count is set.  No counterpart in the reference. */

@@ -170,6 +170,49 @@ def test_concurrent_handles_share_panel_streams_bitwise(gpu):
     assert np.max(np.abs(np.linalg.svd(full, compute_uv=False) - s)) <= bar(np.float64, n) * s[0]
 
 
+def test_phase_timing_switch_per_run(gpu):
+    """avs_set_phase_timing turns the per-launch events on or off for the next
+    runs; the readout describes the last run only and is refused after a run
+    without events; the events change no bit of the band."""
+    import ctypes
+    from avsvd._lib import Opts, check, dptr, lib
+    L = lib()
+    n, b = 512, 64
+    a = np.asfortranarray(O.gaussian_matrix(n, BASE + 41))
+    o = Opts()
+    L.avs_default_opts(ctypes.byref(o))
+    h = ctypes.c_void_p()
+    check(L.avs_create(ctypes.byref(h), avsvd.AVS_F64, n, b, ctypes.byref(o)))
+    try:
+        pm, um, uf = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        pn, un = ctypes.c_int64(), ctypes.c_int64()
+
+        def run(timing):
+            print("case n=%d b=%d phase_timing=%d" % (n, b, timing), flush=True)
+            check(L.avs_set_phase_timing(h, timing))
+            check(L.avs_set_matrix_host(h, a.ctypes.data_as(ctypes.c_void_p), n))
+            check(L.avs_blockbidiag(h, 0))
+            band = np.zeros((b + 1, n), order="F")
+            check(L.avs_get_band(h, dptr(band), b + 1))
+            st = L.avs_phase_timing(h, ctypes.byref(pm), ctypes.byref(pn), ctypes.byref(um), ctypes.byref(un),
+                                    ctypes.byref(uf))
+            return band, st
+
+        b0, st0 = run(0)
+        b1, st1 = run(1)
+        assert st1 == 0 and pn.value > 0 and un.value > 0 and um.value > 0 and uf.value > 0
+        check(L.avs_set_phase_timing(h, 0))
+        st_stale = L.avs_phase_timing(h, ctypes.byref(pm), ctypes.byref(pn), ctypes.byref(um), ctypes.byref(un),
+                                      ctypes.byref(uf))
+        b2, st2 = run(0)
+        assert st0 != 0 and st2 != 0
+        assert st_stale == 0  # the last run (with events) is still the one described
+        np.testing.assert_array_equal(b0, b1)
+        np.testing.assert_array_equal(b0, b2)
+    finally:
+        L.avs_destroy(h)
+
+
 def test_reserved_cus_are_bitwise_neutral(gpu):
     """Updates on a CU-masked stream (16 CUs left to the panels), all of them
     or only the TSQRT updates (the bench configuration), the merges or both, give the same bits as the default

@@ -231,7 +231,12 @@ void copy_parallel(void *dst, const void *src, size_t bytes)
     std::vector<std::thread> th;
     for (int i = 0; i < nt; ++i) {
         const size_t a = std::min(bytes, per * i), e = std::min(bytes, per * (i + 1));
-        if (a < e) th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, e - a); });
+        if (a >= e) continue;
+        try {
+            th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, e - a); });
+        } catch (...) { // no thread to be had: copy this part here
+            memcpy((char *)dst + a, (const char *)src + a, e - a);
+        }
     }
     for (std::thread &t : th) t.join();
 }

@@ -153,7 +153,8 @@ struct avs_handle {
     // panel stream of the stream-ordered executor and one completion event per
     // phase (the panels of a sweep run ahead of that sweep's updates)
     hipStream_t pstream = nullptr;
-    int pslot = -1;  // the panel stream's slot in its device's pool
+    int pslot = -1;  // the pool slot of the last run's panel stream (-1: none, or the handle's own)
+    bool panel_pool = true; // panel streams from the device's pool when AVSVD_PANEL_QUEUES > 0
     std::vector<hipEvent_t> done;
     // update stream whose CU mask leaves opts.reserve_cus CUs to the panels,
     // and the events that join it to the handle's stream
@@ -163,35 +164,67 @@ struct avs_handle {
 
 namespace {
 
-// Panel streams come from a small per-device pool of streams that each own a
-// hardware queue.  The HIP runtime maps ordinary streams onto a few shared
-// hardware queues (GPU_MAX_HW_QUEUES, 4 by default); once a process holds more
-// streams than that, a handle's panel and update streams can land on one
-// queue, where every panel waits behind the updates enqueued before it (n=16384
-// FP64 b=128 on MI355X: 474 ms instead of 358 ms for the handle concerned, the
-// one-stream time).  A stream created with a CU mask gets a queue of its own;
-// the mask here enables every CU.  At most kPanelQueues such streams per
-// device: with a sixth next to the runtime's four queues its handle was slower
-// again (428 ms).  Further handles share the pool's streams, least used first
-// (their panels then serialise with another handle's panels, never with their
-// own updates).  The pool's streams live until the process ends.
-constexpr int kPanelQueues = 4;
+// The panel stream.  By default every handle has an ordinary stream of its
+// own.  The HIP runtime maps ordinary streams onto a few shared hardware
+// queues (GPU_MAX_HW_QUEUES, 4 by default); once a process holds more streams
+// than that, a handle's panel and update streams can land on one queue, where
+// every panel waits behind the updates enqueued before it (measured at
+// n=16384 FP64 b=128, two handles in one process: the second took 471 ms, the
+// one-stream time, instead of 355).  With AVSVD_PANEL_QUEUES=k (1..4) panel
+// streams come instead from a per-device pool of k streams created with a
+// CU mask that enables every CU, each of which owns a hardware queue.  Each run
+// takes an idle pool stream, starting after the one taken last, so up to k runs
+// in flight get a queue each (the two handles above: 355 and 356 ms).  With
+// every pool stream busy, runs share them in turn; their panels then
+// serialise with another run's panels, never with their own updates.  A run's
+// first panel waits for every earlier update of its handle, so runs of one
+// handle stay ordered whichever streams they take.  The pool's streams live
+// until the process ends.
+//
+// The pool is not the default because dedicated queues slow a process that
+// runs many small reductions concurrently, even when they sit idle.  The
+// 48-request serving mix ran 9.8–11.0 solves/s with 1–4 pool queues in the
+// process (its own handles on ordinary streams or not), against 14.1 without
+// any.  Raising GPU_MAX_HW_QUEUES to 8 fixes the two-handle case as well (both
+// 357 ms).  The serving loop's handles always take ordinary streams
+// (handle_use_own_panel_stream).
+// At most 4: a sixth dedicated queue beside the runtime's four made its
+// handle slower again (428 ms).
+constexpr int kPanelQueuesMax = 4;
+int panel_queues()
+{
+    static const int k = [] {
+        const char *e = getenv("AVSVD_PANEL_QUEUES");
+        const int v = e ? atoi(e) : 0;
+        return v < 0 ? 0 : (v > kPanelQueuesMax ? kPanelQueuesMax : v);
+    }();
+    return k;
+}
 struct PanelPool {
     std::vector<hipStream_t> s;
-    std::vector<int> users;
+    int next = 0;
 };
 std::mutex g_ppool_mu;
 std::map<int, PanelPool> g_ppool;
 
+// the panel stream of the run about to be enqueued (a handle's own stream,
+// pslot < 0, is kept)
 avs_status acquire_panel_stream(avs_handle *h)
 {
-    if (h->pstream) return AVS_OK;
+    if (h->pstream && h->pslot < 0) return AVS_OK;
+    if (!h->panel_pool || panel_queues() == 0) {
+        HIPCHK(hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking));
+        return AVS_OK;
+    }
     std::lock_guard<std::mutex> lk(g_ppool_mu);
     PanelPool &P = g_ppool[h->device];
+    const int ns = (int)P.s.size();
     int slot = -1;
-    for (size_t i = 0; i < P.s.size(); ++i)
-        if (slot < 0 || P.users[i] < P.users[slot]) slot = (int)i;
-    if ((slot < 0 || P.users[slot] > 0) && (int)P.s.size() < kPanelQueues) {
+    for (int k = 0; k < ns && slot < 0; ++k) {
+        const int i = (P.next + k) % ns;
+        if (hipStreamQuery(P.s[i]) == hipSuccess) slot = i; // nothing pending
+    }
+    if (slot < 0 && ns < panel_queues()) {
         int ncu = 0;
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
         // the size argument counts 32-bit words; ncu words covers every CU
@@ -200,20 +233,19 @@ avs_status acquire_panel_stream(avs_handle *h)
         hipStream_t st = nullptr;
         HIPCHK(hipExtStreamCreateWithCUMask(&st, (uint32_t)ncu, mask.data()));
         P.s.push_back(st);
-        P.users.push_back(0);
-        slot = (int)P.s.size() - 1;
+        slot = ns;
     }
-    ++P.users[slot];
+    if (slot < 0) slot = P.next % ns;
+    P.next = (slot + 1) % (int)P.s.size();
     h->pstream = P.s[slot];
     h->pslot = slot;
     return AVS_OK;
 }
 
+// the handle stops using the pool's streams
 void release_panel_stream(avs_handle *h)
 {
     if (h->pslot < 0) return;
-    std::lock_guard<std::mutex> lk(g_ppool_mu);
-    --g_ppool[h->device].users[h->pslot];
     h->pslot = -1;
     h->pstream = nullptr;
 }
@@ -618,6 +650,7 @@ avs_status band_enqueue(avs_handle *h, const double **host_band)
 }
 
 void *handle_stream(avs_handle *h) { return h->stream; }
+void handle_use_own_panel_stream(avs_handle *h) { h->panel_pool = false; }
 
 } // namespace avs
 

@@ -33,6 +33,9 @@ bool bisect_on_device(int64_t n);
 avs_status band_enqueue(avs_handle *h, const double **host_band);
 // the handle's stream (hipStream_t)
 void *handle_stream(avs_handle *h);
+// the handle's panel streams are ordinary streams of its own, not the
+// device's pool of dedicated queues (for the serving loop's handles)
+void handle_use_own_panel_stream(avs_handle *h);
 // band -> bidiagonal by Householder bulge chasing, sweeps pipelined over
 // nthreads host threads (bandbrd.cpp); the result does not depend on nthreads
 avs_status bidiagonalize_band_chase(int64_t n, int32_t bw, const double *band, int64_t ldband, double *d, double *e,

@@ -312,6 +312,7 @@ avs_status start(avs_server *S, Slot &x, Running &run)
         avs_default_opts(&o);
         o.update_mode = S->opts.update_mode;
         if ((st = avs_create(&x.h, r.dtype, r.N, r.b, &o))) return st;
+        avs::handle_use_own_panel_stream(x.h);
         if (hipEventCreateWithFlags(&x.ev, hipEventDisableTiming) != hipSuccess)
             return fail(AVS_ERR_HIP, "server: event creation failed");
         const size_t es = r.dtype == AVS_F64 ? 8 : 4;

@@ -122,9 +122,9 @@ def test_panel_stream_overlap_is_bitwise_neutral(gpu, n, b, dt, tree):
 
 def test_concurrent_handles_share_panel_streams_bitwise(gpu):
     """Six handles reduced at once (every run enqueued before any is waited
-    for): panel streams come from a per-device pool of four streams with their
-    own hardware queues, so two pairs of handles share one; every band equals
-    the same matrix reduced by its handle alone."""
+    for), so their twelve streams share the runtime's hardware queues (or, with
+    AVSVD_PANEL_QUEUES set, six runs share the pool's panel queues): every band
+    equals the same matrix reduced by its handle alone."""
     import ctypes
     from avsvd._lib import Opts, check, dptr, lib
     L = lib()

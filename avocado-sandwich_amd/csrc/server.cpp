@@ -512,11 +512,11 @@ void host_worker(avs_server *S)
         r->e.assign(std::max<int64_t>(N - 1, 1), 0.0);
         // one host thread per request, four from n = 4096 and
         // AVSVD_SERVER_BIG_THREADS (default 8) from n = 8192: the largest
-        // requests set the p99 and the makespan, and their sweeps pipeline
-        // well.  24 mixed requests on a 16-core host: 8 threads gave Poisson
-        // p99 1.34 s against 1.69 s with 4, and 9.9 against 9.6 solves/s at
-        // saturation; every host thread from n = 8192 was slower (the
-        // overlapping large requests then oversubscribe the cores).
+        // requests set the p99, and their sweeps pipeline well.  48 mixed
+        // requests on a 16-core host: Poisson p99 1.08-1.20 s with 8 threads
+        // against 1.60-1.70 s with 4 and 0.96 s with 12, at 13.9-14.7 solves/s
+        // saturated against 14.2-15.0 (4) and 13.8 (12); every host thread was
+        // slower still (the overlapping large requests oversubscribe the cores).
         static const int big_threads = [] {
             const char *e = getenv("AVSVD_SERVER_BIG_THREADS");
             const int v = e ? atoi(e) : 0;

@@ -170,6 +170,46 @@ def test_concurrent_handles_share_panel_streams_bitwise(gpu):
     assert np.max(np.abs(np.linalg.svd(full, compute_uv=False) - s)) <= bar(np.float64, n) * s[0]
 
 
+def test_pooled_panel_queues_bitwise(gpu):
+    """AVSVD_PANEL_QUEUES=2 (read once per process, so in a child process):
+    panel streams from a pool of two dedicated hardware queues, taken per run;
+    three handles reduced at once give the bands each gives alone."""
+    import subprocess
+    import sys
+    code = r"""
+import ctypes, numpy as np, avsvd, oracle as O
+from avsvd._lib import Opts, check, dptr, lib
+L = lib(); n, b, k = 512, 64, 3
+mats = [np.asfortranarray(O.gaussian_matrix(n, 20261203 + 71 + i)) for i in range(k)]
+o = Opts(); L.avs_default_opts(ctypes.byref(o))
+hs = []
+for _ in range(k):
+    h = ctypes.c_void_p(); check(L.avs_create(ctypes.byref(h), avsvd.AVS_F64, n, b, ctypes.byref(o))); hs.append(h)
+def bands(together):
+    for h, a in zip(hs, mats):
+        check(L.avs_set_matrix_host(h, a.ctypes.data_as(ctypes.c_void_p), n)); check(L.avs_synchronize(h))
+    for h in hs:
+        check(L.avs_blockbidiag(h, 0))
+        if not together: check(L.avs_synchronize(h))
+    out = []
+    for h in hs:
+        band = np.zeros((b + 1, n), order="F"); check(L.avs_get_band(h, dptr(band), b + 1)); out.append(band)
+    return out
+alone, together = bands(False), bands(True)
+for h in hs: L.avs_destroy(h)
+assert all(np.array_equal(x, y) for x, y in zip(alone, together))
+print("pooled panel queues OK", flush=True)
+"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, AVSVD_PANEL_QUEUES="2",
+               PYTHONPATH=os.pathsep.join([os.path.join(root, "avocado-sandwich_amd"), os.path.join(root, "oracle"),
+                                           root, os.environ.get("PYTHONPATH", "")]))
+    print("case n=512 b=64 handles=3 AVSVD_PANEL_QUEUES=2 (child process)", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "pooled panel queues OK" in r.stdout
+
+
 def test_phase_timing_switch_per_run(gpu):
     """avs_set_phase_timing turns the per-launch events on or off for the next
     runs; the readout describes the last run only and is refused after a run

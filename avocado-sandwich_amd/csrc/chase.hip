@@ -13,6 +13,9 @@
 // outside it is skipped and flags the matrix, which the host reports.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "kernels.hpp"
 
 namespace avs {
@@ -362,18 +365,14 @@ constexpr int WCH_HALF = 64; // elements of a line per thread
 // divisions; tau = 0 and nothing written when the rest of the segment is
 // zero): v to vsh, tau to sc2[0], and the segment becomes (beta, 0, ..., 0).
 // No update line holds the segment, so it is written before the barrier.
-template <bool RIGHT>
-__device__ __forceinline__ void wave0_reflector(const BandStore &A, long p, long c0, int m, double *vsh, double *sc2)
+// The reflector of the m values x held by the lanes of wave 0 (x0 for k =
+// lane, x1 for k = lane + 64; zero past m): v to vsh, tau to sc2[0], and the
+// segment's new values (beta, 0, ..., 0) through put(k, value).  ALWAYS: put
+// is also called when tau = 0, with the unchanged values.
+template <bool ALWAYS, typename Put>
+__device__ __forceinline__ void wave0_house2(double x0, double x1, int m, double *vsh, double *sc2, Put put)
 {
-    const int t = threadIdx.x;
-    if (t >= 64) return;
-    const int k0 = t, k1 = t + 64;
-    auto seg = [&](int k) { return RIGHT ? A.get(p, c0 + k) : A.get(c0 + k, c0); };
-    auto put = [&](int k, double v) {
-        if (RIGHT) A.set(p, c0 + k, v);
-        else A.set(c0 + k, c0, v);
-    };
-    const double x0 = k0 < m ? seg(k0) : 0.0, x1 = k1 < m ? seg(k1) : 0.0;
+    const int k0 = threadIdx.x, k1 = k0 + 64;
     const double alpha = __shfl(x0, 0, 64);
     double sc = wave_max(fmax((k0 >= 1 && k0 < m) ? fabs(x0) : 0.0, k1 < m ? fabs(x1) : 0.0));
     double tau = 0.0;
@@ -392,8 +391,31 @@ __device__ __forceinline__ void wave0_reflector(const BandStore &A, long p, long
             vsh[k1] = x1 / den;
             put(k1, 0.0);
         }
+    } else if (ALWAYS) {
+        if (k0 < m) put(k0, x0);
+        if (k1 < m) put(k1, x1);
     }
-    if (t == 0) sc2[0] = tau;
+    if (k0 == 0) sc2[0] = tau;
+}
+
+// The reflector of a phase, on wave 0 (two elements per lane, m <= 128):
+// RIGHT annihilates row p over columns c0 .. c0+m-1, LEFT column c0 over rows
+// c0 .. c0+m-1.  Householder vector as wave_house (LAPACK dlarfg convention,
+// divisions; tau = 0 and nothing written when the rest of the segment is
+// zero): v to vsh, tau to sc2[0], and the segment becomes (beta, 0, ..., 0).
+// No update line holds the segment, so it is written before the barrier.
+template <bool RIGHT>
+__device__ __forceinline__ void wave0_reflector(const BandStore &A, long p, long c0, int m, double *vsh, double *sc2)
+{
+    const int t = threadIdx.x;
+    if (t >= 64) return;
+    const int k0 = t, k1 = t + 64;
+    auto seg = [&](int k) { return RIGHT ? A.get(p, c0 + k) : A.get(c0 + k, c0); };
+    const double x0 = k0 < m ? seg(k0) : 0.0, x1 = k1 < m ? seg(k1) : 0.0;
+    wave0_house2<false>(x0, x1, m, vsh, sc2, [&](int k, double v) {
+        if (RIGHT) A.set(p, c0 + k, v);
+        else A.set(c0 + k, c0, v);
+    });
 }
 
 // Right phase: the reflector of row p over columns c0 .. c0+m-1, applied to
@@ -490,11 +512,175 @@ __device__ __forceinline__ void wg_left_phase(const BandStore &A, long c0, int m
     __syncthreads();
 }
 
+// T3 + T4 of one step (s >= 1) with the right phase's output for the left
+// phase kept in LDS.  The right reflector (row r over columns ca .. ca+m-1,
+// m = min(b, N - ca)) updates rows r+1 .. ca+m-1; the left reflector (column
+// ca over rows ca .. ca+m-1) then reads and rewrites rows ca .. ca+m-1 of
+// columns ca .. min(ca+2b-1, N-1).  Rows ca .. ca+m-1 of the right phase's
+// columns therefore go to LDS (Lt[k*b + row - ca], column ca + k) instead of
+// a store to HBM and a load back; the left phase writes them to HBM whether
+// or not its reflector is the identity.  Columns ca+m .. of the left phase,
+// which the right phase does not touch, come from HBM.  The arithmetic and its
+// order are those of wg_right_phase + wg_left_phase.
+// CPW: left-phase columns per wave (2b-1 <= 8*CPW); JF: the first j whose
+// column lies beyond the right phase's when m = b (b = 4*CPW).
+template <int CPW, int JF>
+__device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int m, double *vsh, double *sc2,
+                                       double *Lt)
+{
+    constexpr int NW = WCH_THREADS / 64;
+    const long b = A.b, last = A.last;
+    const int ldt = (int)b;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    // ---- right phase loads: two threads per row, rows r+1 .. lendR
+    const long lendR = ca + m - 1; // = min(r + 2b - 1, last)
+    const int line = t >> 1, h = t & 1;
+    const long ln = r + 1 + line;
+    const bool valid = ln <= lendR;
+    const long base = A.index(ln, ca), stride = A.ld - 1;
+    const long klo = max(0L, ln - ca - A.b);
+    const long khi = min(min((long)m - 1, A.last - ca), ln - ca + A.ku);
+    if (valid && (klo > 0 || khi < m - 1)) *A.err = 1;
+    const int kb = max((int)klo, WCH_HALF * h), ke = min((int)khi, WCH_HALF * h + WCH_HALF - 1);
+    const bool mine = valid && kb <= ke;
+    double a[WCH_HALF];
+    if (mine) {
+        const double *src = A.S + base;
+#pragma unroll
+        for (int u = 0; u < WCH_HALF; ++u) a[u] = src[(long)(kb + u) * stride];
+    }
+    // ---- right reflector (row r's segment written to HBM)
+    wave0_reflector<true>(A, r, ca, m, vsh, sc2);
+    __syncthreads();
+    {
+        const double tau = sc2[0];
+        double w = 0.0;
+        if (tau != 0.0 && mine) {
+#pragma unroll
+            for (int u = 0; u < WCH_HALF; ++u) w = kb + u <= ke ? fma(a[u], vsh[min(kb + u, ke)], w) : w;
+        }
+        const double o = __shfl_xor(w, 1, 64);
+        const double f = tau * (h == 0 ? w + o : o + w);
+        if (mine) {
+            if (tau != 0.0) {
+#pragma unroll
+                for (int u = 0; u < WCH_HALF; ++u)
+                    if (kb + u <= ke) a[u] = fma(-f, vsh[kb + u], a[u]);
+            }
+            if (ln >= ca) {
+                double *dst = Lt + (ln - ca);
+#pragma unroll
+                for (int u = 0; u < WCH_HALF; ++u)
+                    if (kb + u <= ke) dst[(kb + u) * ldt] = a[u];
+            } else if (tau != 0.0) {
+                double *dst = A.S + base;
+#pragma unroll
+                for (int u = 0; u < WCH_HALF; ++u)
+                    if (kb + u <= ke) dst[(long)(kb + u) * stride] = a[u];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- left phase: columns ca+1+wv+NW*j.  Those of the right phase (offset
+    // < m) are read from LDS when used; the rest, untouched by it, come from
+    // HBM: for j >= JF (all of them when m = b = 4*CPW) requested now, while wave 0
+    // forms the reflector from column ca in LDS (written to HBM in any case)
+    const long lendL = min(ca + 2 * b - 1, last);
+    double ff[CPW - JF][2];
+#pragma unroll
+    for (int j = JF; j < CPW; ++j) {
+        const long c = ca + 1 + wv + (long)NW * j;
+        ff[j - JF][0] = ff[j - JF][1] = 0.0;
+        if (c <= lendL && c - ca >= m) { // wave-uniform
+            const double *src = A.S + A.index(ca, c);
+            ff[j - JF][0] = src[lane];
+            ff[j - JF][1] = src[lane + 64];
+        }
+    }
+    if (t < 64) {
+        const double x0 = lane < m ? Lt[lane] : 0.0, x1 = lane + 64 < m ? Lt[lane + 64] : 0.0;
+        wave0_house2<true>(x0, x1, m, vsh, sc2, [&](int k, double v) { A.set(ca + k, ca, v); });
+    }
+    __syncthreads();
+    const double tau = sc2[0];
+    // column j's values: LDS (offset < m; Lt holds b*b + 128 doubles, so lane
+    // + 64 of column m-1 stays inside it), the early HBM request, or HBM now
+    auto colv = [&](int j, double &x0, double &x1) {
+        const long c = ca + 1 + wv + (long)NW * j;
+        const int off = (int)(c - ca);
+        if (j >= JF && off >= m) {
+            x0 = ff[j - JF][0];
+            x1 = ff[j - JF][1];
+        } else if (off < m) {
+            x0 = Lt[off * ldt + lane];
+            x1 = Lt[off * ldt + lane + 64];
+        } else {
+            const double *src = A.S + A.index(ca, c);
+            x0 = src[lane];
+            x1 = src[lane + 64];
+        }
+    };
+    if (tau != 0.0) {
+        const double v0 = vsh[lane], v1 = vsh[lane + 64];
+        // the columns' dot products, 8 at a time with no branch between them,
+        // so their butterflies (wave_sum's steps, level by level) overlap
+        constexpr int G = 8;
+#pragma unroll
+        for (int g = 0; g < CPW; g += G) {
+            if (ca + 1 + wv + (long)NW * g > lendL) break; // wave-uniform
+            double x[G][2], pr[G];
+#pragma unroll
+            for (int jj = 0; jj < G; ++jj) {
+                const long c = ca + 1 + wv + (long)NW * (g + jj);
+                const long klo2 = max(max(0L, c - A.ku - ca), -ca);
+                const long khi2 = min(min((long)m - 1, c + A.b - ca), A.last - ca);
+                const bool ok = c <= lendL;
+                x[jj][0] = x[jj][1] = 0.0;
+                if (ok) colv(g + jj, x[jj][0], x[jj][1]);
+                const bool in0 = ok && lane >= klo2 && lane <= khi2;
+                const bool in1 = ok && lane + 64 >= klo2 && lane + 64 <= khi2;
+                const double p = in0 ? x[jj][0] * v0 : 0.0;
+                pr[jj] = in1 ? fma(x[jj][1], v1, p) : p;
+            }
+#pragma unroll
+            for (int mm = 32; mm >= 1; mm >>= 1) {
+#pragma unroll
+                for (int jj = 0; jj < G; ++jj) pr[jj] += __shfl_xor(pr[jj], mm, 64);
+            }
+#pragma unroll
+            for (int jj = 0; jj < G; ++jj) {
+                const long c = ca + 1 + wv + (long)NW * (g + jj);
+                if (c > lendL) continue; // wave-uniform
+                const long klo2 = max(max(0L, c - A.ku - ca), -ca);
+                const long khi2 = min(min((long)m - 1, c + A.b - ca), A.last - ca);
+                if (lane == 0 && (klo2 > 0 || khi2 < m - 1)) *A.err = 1;
+                const bool in0 = lane >= klo2 && lane <= khi2, in1 = lane + 64 >= klo2 && lane + 64 <= khi2;
+                const double f = tau * pr[jj];
+                double *dst = A.S + A.index(ca, c);
+                if (in0) dst[lane] = fma(-f, v0, x[jj][0]);
+                if (in1) dst[lane + 64] = fma(-f, v1, x[jj][1]);
+            }
+        }
+    } else {
+        // the identity: the right phase's rows still go to HBM
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            const long c = ca + 1 + wv + (long)NW * j;
+            if (c > lendL || c - ca >= m) continue; // wave-uniform
+            const int off = (int)(c - ca);
+            double *dst = A.S + A.index(ca, c);
+            if (lane < m) dst[lane] = Lt[off * ldt + lane];
+            if (lane + 64 < m) dst[lane + 64] = Lt[off * ldt + lane + 64];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(WCH_THREADS) chase_step_kernel(double *store, long N, int bw, long tstep, long ihi,
-                                                                 int *err)
+                                                                 int *err, int lds)
 {
     __shared__ double vsh[2 * WCH_HALF];
     __shared__ double sc2[2];
+    extern __shared__ double Lt[]; // b*b + 128: the T3 -> T4 rows of a step
     const long b = bw;
     BandStore A;
     A.ld = 3 * b + 1;
@@ -523,6 +709,13 @@ __global__ void __launch_bounds__(WCH_THREADS) chase_step_kernel(double *store, 
     const long ca = r + b, cb = min(r + 2 * b - 1, last);
     if (cb <= ca) {
         if (threadIdx.x == 0) *err = 1; // the host schedules only existing steps
+        return;
+    }
+    if (lds) {
+        const int m = (int)(cb - ca + 1);
+        if (b <= 32) wg_t34<8, 3>(A, r, ca, m, vsh, sc2, Lt);
+        else if (b <= 64) wg_t34<16, 7>(A, r, ca, m, vsh, sc2, Lt);
+        else wg_t34<32, 15>(A, r, ca, m, vsh, sc2, Lt);
         return;
     }
     wg_right_phase(A, r, ca, (int)(cb - ca + 1), min(r + 2 * b - 1, last), vsh, sc2);
@@ -560,8 +753,21 @@ void launch_batch_bisect(const double *de, const long *col0, const int *nts, int
 
 void launch_chase_step(double *store, long N, int b, long tstep, long ilo, long ihi, int *err, hipStream_t s)
 {
-    hipLaunchKernelGGL(chase_step_kernel, dim3((unsigned)(ihi - ilo + 1)), dim3(WCH_THREADS), 0, s, store, N, b,
-                       tstep, ihi, err);
+    // T3 -> T4 through LDS when b*b + 128 doubles fit beside the static LDS
+    // (AVSVD_CHASE_LDS=0: through HBM; read once per process)
+    static const long lds_max = [] {
+        const char *e = getenv("AVSVD_CHASE_LDS");
+        if (e && strcmp(e, "0") == 0) return 0L;
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return 0L;
+        return (long)v - 4096;
+    }();
+    const size_t lds = ((size_t)b * b + 128) * sizeof(double);
+    const int use = (long)lds <= lds_max ? 1 : 0;
+    hipLaunchKernelGGL(chase_step_kernel, dim3((unsigned)(ihi - ilo + 1)), dim3(WCH_THREADS), use ? lds : 0, s,
+                       store, N, b, tstep, ihi, err, use);
 }
 
 void launch_chase_diag(const double *store, long N, int b, double *de, hipStream_t s)

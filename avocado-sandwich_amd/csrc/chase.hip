@@ -37,6 +37,51 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// Wavefront reductions on the VALU, with no LDS traffic: within a row of 16
+// lanes by DPP (quad_perm xor 1, xor 2, then row_half_mirror and row_mirror,
+// which pair lanes of equal partial values), across the four rows by
+// v_permlane16_swap / v_permlane32_swap.  Every step adds a lane's value and
+// its partner's, the same two values in every lane of a pair, so all 64 lanes
+// end with the same bits.
+template <int CTRL> __device__ __forceinline__ double dpp_d(double v)
+{
+    return __builtin_amdgcn_update_dpp(0.0, v, CTRL, 0xf, 0xf, true);
+}
+struct PairD { double x, y; };
+template <int W> __device__ __forceinline__ PairD swap_rows_d(double v)
+{
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    if constexpr (W == 16) {
+        const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        return {__hiloint2double((int)rh[0], (int)rl[0]), __hiloint2double((int)rh[1], (int)rl[1])};
+    } else {
+        const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        return {__hiloint2double((int)rh[0], (int)rl[0]), __hiloint2double((int)rh[1], (int)rl[1])};
+    }
+}
+__device__ __forceinline__ double wave_sum_v(double v)
+{
+    v += dpp_d<0xB1>(v);  // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);  // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v); // row_half_mirror
+    v += dpp_d<0x140>(v); // row_mirror
+    const PairD a = swap_rows_d<16>(v);
+    const PairD b = swap_rows_d<32>(a.x + a.y);
+    return b.x + b.y;
+}
+__device__ __forceinline__ double wave_max_v(double v)
+{
+    v = fmax(v, dpp_d<0xB1>(v));
+    v = fmax(v, dpp_d<0x4E>(v));
+    v = fmax(v, dpp_d<0x141>(v));
+    v = fmax(v, dpp_d<0x140>(v));
+    const PairD a = swap_rows_d<16>(v);
+    const PairD b = swap_rows_d<32>(fmax(a.x, a.y));
+    return fmax(b.x, b.y);
+}
+
 struct BandStore {
     double *S;
     long ld, ku, b, last;
@@ -374,12 +419,12 @@ __device__ __forceinline__ void wave0_house2(double x0, double x1, int m, double
 {
     const int k0 = threadIdx.x, k1 = k0 + 64;
     const double alpha = __shfl(x0, 0, 64);
-    double sc = wave_max(fmax((k0 >= 1 && k0 < m) ? fabs(x0) : 0.0, k1 < m ? fabs(x1) : 0.0));
+    double sc = wave_max_v(fmax((k0 >= 1 && k0 < m) ? fabs(x0) : 0.0, k1 < m ? fabs(x1) : 0.0));
     double tau = 0.0;
     if (sc != 0.0) {
         sc = fmax(sc, fabs(alpha));
         const double u0 = x0 / sc, u1 = x1 / sc;
-        const double nrm = sc * sqrt(wave_sum(u0 * u0 + u1 * u1));
+        const double nrm = sc * sqrt(wave_sum_v(u0 * u0 + u1 * u1));
         const double beta = alpha >= 0.0 ? -nrm : nrm;
         tau = (beta - alpha) / beta;
         const double den = alpha - beta;
@@ -524,10 +569,16 @@ __device__ __forceinline__ void wg_left_phase(const BandStore &A, long c0, int m
 // order are those of wg_right_phase + wg_left_phase.
 // CPW: left-phase columns per wave (2b-1 <= 8*CPW); JF: the first j whose
 // column lies beyond the right phase's when m = b (b = 4*CPW).
+// timing hooks of wg_t34 (event ev); empty unless a profiling build defines them
+#ifndef AVS_CHASE_MARK
+#define AVS_CHASE_MARK0
+#define AVS_CHASE_MARK(ev)
+#endif
 template <int CPW, int JF>
 __device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int m, double *vsh, double *sc2,
                                        double *Lt)
 {
+    AVS_CHASE_MARK0;
     constexpr int NW = WCH_THREADS / 64;
     const long b = A.b, last = A.last;
     const int ldt = (int)b;
@@ -551,7 +602,9 @@ __device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int 
     }
     // ---- right reflector (row r's segment written to HBM)
     wave0_reflector<true>(A, r, ca, m, vsh, sc2);
+    AVS_CHASE_MARK(1);
     __syncthreads();
+    AVS_CHASE_MARK(2);
     {
         const double tau = sc2[0];
         double w = 0.0;
@@ -580,7 +633,9 @@ __device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int 
             }
         }
     }
+    AVS_CHASE_MARK(3);
     __syncthreads();
+    AVS_CHASE_MARK(4);
     // ---- left phase: columns ca+1+wv+NW*j.  Those of the right phase (offset
     // < m) are read from LDS when used; the rest, untouched by it, come from
     // HBM: for j >= JF (all of them when m = b = 4*CPW) requested now, while wave 0
@@ -601,7 +656,9 @@ __device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int 
         const double x0 = lane < m ? Lt[lane] : 0.0, x1 = lane + 64 < m ? Lt[lane + 64] : 0.0;
         wave0_house2<true>(x0, x1, m, vsh, sc2, [&](int k, double v) { A.set(ca + k, ca, v); });
     }
+    AVS_CHASE_MARK(5);
     __syncthreads();
+    AVS_CHASE_MARK(6);
     const double tau = sc2[0];
     // column j's values: LDS (offset < m; Lt holds b*b + 128 doubles, so lane
     // + 64 of column m-1 stays inside it), the early HBM request, or HBM now
@@ -622,9 +679,9 @@ __device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int 
     };
     if (tau != 0.0) {
         const double v0 = vsh[lane], v1 = vsh[lane + 64];
-        // the columns' dot products, 8 at a time with no branch between them,
-        // so their butterflies (wave_sum's steps, level by level) overlap
-        constexpr int G = 8;
+        // the columns' dot products, 4 at a time with no branch between them,
+        // so their reductions overlap
+        constexpr int G = 4;
 #pragma unroll
         for (int g = 0; g < CPW; g += G) {
             if (ca + 1 + wv + (long)NW * g > lendL) break; // wave-uniform
@@ -643,10 +700,7 @@ __device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int 
                 pr[jj] = in1 ? fma(x[jj][1], v1, p) : p;
             }
 #pragma unroll
-            for (int mm = 32; mm >= 1; mm >>= 1) {
-#pragma unroll
-                for (int jj = 0; jj < G; ++jj) pr[jj] += __shfl_xor(pr[jj], mm, 64);
-            }
+            for (int jj = 0; jj < G; ++jj) pr[jj] = wave_sum_v(pr[jj]);
 #pragma unroll
             for (int jj = 0; jj < G; ++jj) {
                 const long c = ca + 1 + wv + (long)NW * (g + jj);
@@ -673,6 +727,7 @@ __device__ __forceinline__ void wg_t34(const BandStore &A, long r, long ca, int 
             if (lane + 64 < m) dst[lane + 64] = Lt[off * ldt + lane + 64];
         }
     }
+    AVS_CHASE_MARK(7);
 }
 
 __global__ void __launch_bounds__(WCH_THREADS) chase_step_kernel(double *store, long N, int bw, long tstep, long ihi,

@@ -342,26 +342,29 @@ __global__ void __launch_bounds__(BIS_THREADS) batch_bisect_kernel(const double 
 }
 
 // One bidiagonal's singular values by the same Sturm counts, t^2 read from
-// HBM (every lane reads the same t2[k] at the same step), four lanes per
-// singular value: in each of 32 rounds lanes 0..2 of the group count at the
-// quarter points of [lo, hi], the counts meet through lane exchanges, and all
-// four lanes keep the same quarter (4^-32 = 2^-64 of the Gershgorin bound, as
-// 64 halvings, in half the sequential rounds and on four times the lanes).
-// Group j of the grid finds the j-th smallest; the nkeep largest are written,
-// descending.
-constexpr int BIS_LANES = 4;
+// HBM (every lane reads the same t2[k] at the same step), sixteen lanes per
+// singular value: in each of 16 rounds lanes 0..14 of the group count at the
+// points lo + (k+1)/16 (hi - lo), the group's counts meet in one ballot, and
+// all sixteen lanes keep the same sixteenth (16^-16 = 2^-64 of the
+// Gershgorin bound, as 64 halvings, in a quarter of the sequential rounds and
+// on sixteen times the lanes, so four wavefronts per SIMD at n = 16384 hide
+// the latency of the dependent divisions).  Group j of the grid finds the
+// j-th smallest; the nkeep largest are written, descending.
+constexpr int BIS_LANES = 16;
 __global__ void __launch_bounds__(BIS_THREADS) bisect_kernel(const double *__restrict__ t2, long N, long nkeep,
                                                              double hi0, double pivmin, double *__restrict__ sigma)
 {
     const long g = (long)blockIdx.x * BIS_THREADS + threadIdx.x;
-    const long j = g / BIS_LANES; // the same for the four lanes of a group
+    const long j = g / BIS_LANES; // the same for the lanes of a group
     const int sub = (int)(g % BIS_LANES);
+    // groups are 16-lane aligned, so a group returns or stays as a whole
     if (j >= N || j < N - nkeep) return;
+    const int base = (int)(threadIdx.x & 63) & ~(BIS_LANES - 1);
     const long L = 2 * N - 1;
     double lo = 0.0, hi = hi0;
-    for (int it = 0; it < 32; ++it) {
+    for (int it = 0; it < 16; ++it) {
         const double w = hi - lo;
-        const double x = lo + w * (0.25 * (double)(min(sub, 2) + 1));
+        const double x = lo + w * ((double)(min(sub, 14) + 1) * 0.0625);
         double q = -x;
         if (fabs(q) < pivmin) q = -pivmin;
         long cnt = q < 0.0;
@@ -370,17 +373,15 @@ __global__ void __launch_bounds__(BIS_THREADS) bisect_kernel(const double *__res
             if (fabs(q) < pivmin) q = -pivmin;
             cnt += q < 0.0;
         }
-        // quarter point s lies at or above the j-th smallest value when more
-        // than j singular values are below it
+        // point k lies at or above the j-th smallest value when more than j
+        // singular values are below it; the first such point ends the interval
         const bool above = cnt - N >= j + 1;
-        const bool a0 = __shfl((int)above, 0, BIS_LANES) != 0;
-        const bool a1 = __shfl((int)above, 1, BIS_LANES) != 0;
-        const bool a2 = __shfl((int)above, 2, BIS_LANES) != 0;
-        const double x0 = lo + w * 0.25, x1 = lo + w * 0.5, x2 = lo + w * 0.75;
-        if (a0) hi = x0;
-        else if (a1) { lo = x0; hi = x1; }
-        else if (a2) { lo = x1; hi = x2; }
-        else lo = x2;
+        const unsigned long long bal = __ballot(above);
+        const unsigned bits = (unsigned)(bal >> base) & 0x7fffu;
+        const int k = bits ? __builtin_ctz(bits) : 15;
+        const double nlo = k == 0 ? lo : lo + w * ((double)k * 0.0625);
+        if (k < 15) hi = lo + w * ((double)(k + 1) * 0.0625);
+        lo = nlo;
     }
     if (sub == 0) sigma[N - 1 - j] = 0.5 * (lo + hi);
 }

@@ -677,16 +677,18 @@ avs_status chase_on_device(avs_handle *h, double *d, double *e)
 
 // Where avs_svdvals runs the band -> bidiagonal stage: AVSVD_STAGE2=device on
 // the device; any other value (host, halving, dgbbrd) on the host; unset, on
-// the device from n = AVSVD_DEVICE_CHASE_MIN_N (default 16384) up.  Each time
-// of the device schedule is one launch of about 15 us (b = 32) to 45 us
-// (b = 128) on MI355X, and there are about 3n of them, so below n = 16384 the
-// host chase on 16 threads is faster (n = 16384, after the reduction: 1.03 vs
-// 1.10 s at b = 32, 1.41 vs 1.70 s at b = 64, 2.31 vs 2.80 s at b = 128).
+// the device from n = AVSVD_DEVICE_CHASE_MIN_N up (default 8192 for b > 64,
+// 16384 otherwise).  Each time of the device schedule is one launch, about 3n
+// of them, of 15 us (b = 32) to 36 us (b = 128) on MI355X, and the host
+// chase runs on 16 threads.  After the reduction, device vs host seconds
+// (device bisection in both): b = 128: 0.65 vs 0.68 at n = 6144, 0.88 vs 0.97
+// at 8192, 1.36 vs 1.80 at 12288, 1.86 vs 2.9 at 16384; b = 64: 0.59 vs 0.43
+// at 8192, 0.91 vs 0.90 at 12288, 1.25 vs 1.68 at 16384.
 bool svdvals_chase_on_device(int64_t n, int b)
 {
     if (b < 2 || b > 128) return false;
     if (const char *p = getenv("AVSVD_STAGE2")) return strcmp(p, "device") == 0;
-    int64_t nmin = 16384;
+    int64_t nmin = b > 64 ? 8192 : 16384;
     if (const char *p = getenv("AVSVD_DEVICE_CHASE_MIN_N")) nmin = atoll(p);
     return n >= nmin;
 }

@@ -191,7 +191,8 @@ avs_status avs_get_band(avs_handle *h, double *band, int64_t ldband);
  *     Blockbidiag! on the reference's array would).
  *  2. Band -> bidiagonal in FP64 by Householder bulge chasing: on the device
  *     (avs_bidiagonalize_band_device, no download of the band) for
- *     n >= AVSVD_DEVICE_CHASE_MIN_N (default 16384) and 2 <= b <= 128, otherwise on
+ *     n >= AVSVD_DEVICE_CHASE_MIN_N (default 8192 for b > 64, else 16384) and
+ *     2 <= b <= 128, otherwise on
  *     the host after avs_get_band (pipelined over the library's host threads).
  *     AVSVD_STAGE2=device | host force either side; AVSVD_STAGE2=halving runs the
  *     reference's block sweeps (bulgechasing.jl:33-42) and =dgbbrd LAPACK's chase.
@@ -246,8 +247,9 @@ avs_status avs_bidiagonalize_band_chase(int64_t n, int32_t bw, const double *ban
  * disjoint rows or columns, so every element sees its updates in sweep
  * order, as on the host); only d and e come back.  Same rounding class as the
  * host chase, not the same bits.  avs_svdvals runs this stage on the device
- * for n >= AVSVD_DEVICE_CHASE_MIN_N (default 16384, where it matches or beats
- * the host chase on 16 threads) and 2 <= b <= 128, or always with
+ * for n >= AVSVD_DEVICE_CHASE_MIN_N (default 8192 for b > 64, else 16384:
+ * where it matches or beats the host chase on 16 threads) and 2 <= b <= 128,
+ * or always with
  * AVSVD_STAGE2=device; AVSVD_STAGE2=host (or halving, dgbbrd) keeps it on the
  * host.  No reference counterpart.  Argument errors as
  * avs_bidiagonalize_band; -2 also for bw > 128. */

@@ -117,6 +117,22 @@ struct BandStore {
 // lane; 0 when x[1 .. m-1] is zero), sets beta, and leaves v in vsh[0 .. m-1]
 // (v[0] = 1) behind a barrier.  Divisions, not reciprocals: the chased bulges
 // decay, and a subnormal scale has no finite reciprocal.
+// The per-wave helpers below synchronise the lanes of one wavefront: with a
+// workgroup barrier in a one-wave workgroup (WG), or, when each wavefront of a
+// larger workgroup runs its own step (!WG), with a wavefront-scope fence and
+// wave barrier (no workgroup barrier may be skipped by some waves only).
+template <bool WG> __device__ __forceinline__ void lanes_sync()
+{
+    if constexpr (WG) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+template <bool WG = true>
 __device__ double wave_house(double x, int m, int l, double &beta, double *vsh)
 {
     const double alpha = __shfl(x, 0, 64);
@@ -132,7 +148,7 @@ __device__ double wave_house(double x, int m, int l, double &beta, double *vsh)
     const double tau = (beta - alpha) / beta;
     const double den = alpha - beta;
     if (l < m) vsh[l] = l == 0 ? 1.0 : x / den;
-    __syncthreads();
+    lanes_sync<WG>();
     return tau;
 }
 
@@ -191,28 +207,30 @@ __device__ void apply_left(const BandStore &A, long r0, int m, double tau, long 
 
 // annihilate row `row` over columns c0 .. c0+m-1 from the right, applying the
 // reflector to rows r0 .. r1
+template <bool WG = true>
 __device__ void right_phase(const BandStore &A, long row, long c0, int m, long r0, long r1, int l, double *vsh)
 {
     double beta;
     const double x = l < m ? A.get(row, c0 + l) : 0.0;
-    const double tau = wave_house(x, m, l, beta, vsh);
+    const double tau = wave_house<WG>(x, m, l, beta, vsh);
     if (tau == 0.0) return;
     apply_right(A, c0, m, tau, r0, r1, l, vsh);
     if (l < m) A.set(row, c0 + l, l == 0 ? beta : 0.0);
-    __syncthreads();
+    lanes_sync<WG>();
 }
 
 // annihilate column `col` over rows col .. col+m-1 from the left, applying
 // the reflector to columns col+1 .. c1
+template <bool WG = true>
 __device__ void left_phase(const BandStore &A, long col, int m, long c1, int l, double *vsh)
 {
     double beta;
     const double x = l < m ? A.get(col + l, col) : 0.0;
-    const double tau = wave_house(x, m, l, beta, vsh);
+    const double tau = wave_house<WG>(x, m, l, beta, vsh);
     if (tau == 0.0) return;
     apply_left(A, col, m, tau, col + 1, c1, l, vsh);
     if (l < m) A.set(col + l, col, l == 0 ? beta : 0.0);
-    __syncthreads();
+    lanes_sync<WG>();
 }
 
 template <typename TS>
@@ -268,6 +286,83 @@ __global__ void __launch_bounds__(CHASE_THREADS) batch_chase_kernel(const TS *__
     // d at de[2 col0 .. + N), e at de[2 col0 + N .. + N - 1)
     double *d = de + 2 * col0[m];
     for (long c = l; c < N; c += CHASE_THREADS) {
+        d[c] = A.get(c, c);
+        if (c + 1 < N) d[N + c] = A.get(c, c + 1);
+    }
+}
+
+// The same chase with one workgroup of CHASE_WAVES wavefronts per matrix and
+// the pipelined schedule of the single-matrix chase: step s of sweep i runs at
+// time 3i + s (s = 0: T1 + T2; s >= 1: T3 + T4 at r = i+1+(s-1)b), the steps
+// of one time touch disjoint rows or columns, and wavefront w takes the steps
+// of sweeps ihi - w, ihi - w - CHASE_WAVES, ... of each time.  Each step is
+// the one-wavefront code above with wavefront-local synchronisation; one
+// workgroup barrier per time (the same count in every wavefront) orders the
+// times.  Every element sees its updates in sweep order, so d and e are those
+// of the one-wavefront kernel, bit for bit.
+constexpr int CHASE_WAVES = 8;
+template <typename TS>
+__global__ void __launch_bounds__(CHASE_WAVES * 64) batch_chase_wg_kernel(const TS *__restrict__ arena,
+                                                                        const long *__restrict__ off,
+                                                                        const int *__restrict__ nts,
+                                                                        const int *__restrict__ order,
+                                                                        const long *__restrict__ col0, int bw,
+                                                                        double *store, double *de, int *err)
+{
+    __shared__ double vsh_all[CHASE_WAVES * 64];
+    constexpr int NT = CHASE_WAVES * 64;
+    const int m = order[blockIdx.x];
+    const int t0 = threadIdx.x, w = t0 >> 6, l = t0 & 63;
+    double *vsh = vsh_all + 64 * w;
+    const long b = bw, nt = nts[m], N = nt * b;
+    BandStore A;
+    A.ld = 3 * b + 1;
+    A.ku = 2 * b;
+    A.b = b;
+    A.last = N - 1;
+    A.S = store + col0[m] * A.ld;
+    A.err = err + m;
+    const TS *tiles = arena + off[m];
+    for (long idx = t0; idx < A.ld * N; idx += NT) {
+        const long c = idx / A.ld, r = c - A.ku + idx % A.ld;
+        double v = 0.0;
+        if (r >= 0 && r >= c - b && r <= c)
+            v = (double)tiles[((c / b) * nt + r / b) * b * b + (r % b) + (c % b) * b];
+        A.S[idx] = v;
+    }
+    __syncthreads();
+    const long last = A.last;
+    // steps of sweep i: T1 + T2, then T3 + T4 for r = i+1, i+1+b, ... with r + b < last
+    auto nsteps = [&](long i) {
+        const long x = last - i - 1 - b;
+        return 1 + (x > 0 ? (x + b - 1) / b : 0);
+    };
+    // the last sweep (i = last - 1) has one step, at time 3(last - 1)
+    const long tend = last >= 1 ? 3 * (last - 1) + 1 : 0;
+    for (long t = 0; t < tend; ++t) {
+        const long ihi = min(t / 3, last - 1);
+        for (long k = w;; k += CHASE_WAVES) {
+            const long i = ihi - k;
+            if (i < 0) break;
+            const long st = t - 3 * i;
+            if (st >= nsteps(i)) break; // finished sweeps form a prefix
+            if (st == 0) {
+                const long c0 = i + 1, c1 = min(i + b, last);
+                if (c1 > c0) right_phase<false>(A, i, c0, (int)(c1 - c0 + 1), i + 1, min(i + b, last), l, vsh);
+                const long col = i + 1, r1 = min(i + b, last);
+                if (r1 > col) left_phase<false>(A, col, (int)(r1 - col + 1), min(col + 2 * b - 1, last), l, vsh);
+            } else {
+                const long r = i + 1 + (st - 1) * b;
+                const long ca = r + b, cb = min(r + 2 * b - 1, last);
+                right_phase<false>(A, r, ca, (int)(cb - ca + 1), r + 1, min(r + 2 * b - 1, last), l, vsh);
+                const long r1 = min(ca + b - 1, last);
+                if (r1 > ca) left_phase<false>(A, ca, (int)(r1 - ca + 1), min(ca + 2 * b - 1, last), l, vsh);
+            }
+        }
+        __syncthreads();
+    }
+    double *d = de + 2 * col0[m];
+    for (long c = t0; c < N; c += NT) {
         d[c] = A.get(c, c);
         if (c + 1 < N) d[N + c] = A.get(c, c + 1);
     }
@@ -835,6 +930,17 @@ template <typename TS>
 void launch_batch_chase(const void *arena, const long *off, const int *nts, const int *order, const long *col0,
                         int count, int b, double *store, double *de, int *err, hipStream_t s)
 {
+    // AVSVD_BATCH_CHASE=wave: one wavefront per matrix, sweeps in sequence
+    // (read once per process); default: a workgroup per matrix, pipelined
+    static const bool one_wave = [] {
+        const char *e = getenv("AVSVD_BATCH_CHASE");
+        return e && strcmp(e, "wave") == 0;
+    }();
+    if (!one_wave) {
+        hipLaunchKernelGGL(batch_chase_wg_kernel<TS>, dim3((unsigned)count), dim3(CHASE_WAVES * 64), 0, s,
+                           (const TS *)arena, off, nts, order, col0, b, store, de, err);
+        return;
+    }
     hipLaunchKernelGGL(batch_chase_kernel<TS>, dim3((unsigned)count), dim3(CHASE_THREADS), 0, s,
                        (const TS *)arena, off, nts, order, col0, b, store, de, err);
 }

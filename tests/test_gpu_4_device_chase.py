@@ -70,3 +70,52 @@ def test_svdvals_device_chase_matches_host_chase(gpu, monkeypatch, n, b, dt):
     ref = sl.svdvals(a.astype(dt).astype(np.float64))
     for o in outs:
         assert O.sigma_err(o, ref) <= 10 * n * np.finfo(dt).eps
+
+
+def test_chase_variants_child_process(gpu, tmp_path):
+    """AVSVD_CHASE_LDS=0 (each step's right-phase rows through HBM instead of
+    LDS) and AVSVD_BATCH_CHASE=wave (one wavefront per batched matrix instead
+    of a pipelined workgroup) are read once per process, so they run in a
+    child process.  The single-matrix chase keeps sigma (its reductions take a
+    different order there); the batched chase gives the same bits."""
+    import hashlib
+    import os
+    import subprocess
+    import sys
+    code = r"""
+import hashlib, numpy as np, avsvd, oracle as O
+n, bw = 1500, 128
+a = np.triu(np.tril(O.gaussian_matrix(n, 20261211 + 3), bw))
+d, e = avsvd.tiled.bidiagonalize_band_device(avsvd.tiled._dense_to_band(a, bw), n, bw)
+np.save(OUT + "_sig.npy", avsvd.bdsdc_values(d, e))
+ns = [33, 200, 300, 1000]
+rng = np.random.default_rng(29)
+bt = avsvd.Batch(ns, dtype=np.float64, b=32)
+for i, k in enumerate(ns):
+    bt.set(i, rng.standard_normal((k, k)) / np.sqrt(k))
+print("batch", hashlib.sha256(b"".join(s.tobytes() for s in bt.svdvals())).hexdigest(), flush=True)
+"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = str(tmp_path / "chase_variants")
+    env = dict(os.environ, AVSVD_CHASE_LDS="0", AVSVD_BATCH_CHASE="wave",
+               PYTHONPATH=os.pathsep.join([os.path.join(root, "avocado-sandwich_amd"), os.path.join(root, "oracle"),
+                                           root, os.environ.get("PYTHONPATH", "")]))
+    print("case chase variants n=1500 bw=128, batch of 4 (child process)", flush=True)
+    r = subprocess.run([sys.executable, "-c", "OUT = %r\n" % out + code], env=env, capture_output=True, text=True,
+                       timeout=110)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    child_batch = [x for x in r.stdout.splitlines() if x.startswith("batch ")][0].split()[1]
+    # the same in this process with the defaults
+    n, bw = 1500, 128
+    a = np.triu(np.tril(O.gaussian_matrix(n, 20261211 + 3), bw))
+    d, e = avsvd.tiled.bidiagonalize_band_device(avsvd.tiled._dense_to_band(a, bw), n, bw)
+    sig = avsvd.bdsdc_values(d, e)
+    assert O.sigma_err(sig, np.load(out + "_sig.npy")) < 10 * n * np.finfo(float).eps
+    assert O.sigma_err(sig, sl.svdvals(a)) < 10 * n * np.finfo(float).eps
+    ns = [33, 200, 300, 1000]
+    rng = np.random.default_rng(29)
+    bt = avsvd.Batch(ns, dtype=np.float64, b=32)
+    for i, k in enumerate(ns):
+        bt.set(i, rng.standard_normal((k, k)) / np.sqrt(k))
+    here = hashlib.sha256(b"".join(s.tobytes() for s in bt.svdvals())).hexdigest()
+    assert here == child_batch
